@@ -1,0 +1,134 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the coupled MRT-LBM step on one GPU.
+
+Default workload: the 3D cubic cavity, 256^3 nodes, D3Q19 + D3Q7, Ra = 1e5,
+Pr = 0.71, erf grid with A = 3.2, FP32 shifted populations, starting from
+the reference initial condition (rest, linear temperature).  A "step" is one
+time step of the whole lattice (one fused kernel launch).
+
+Prints ONE JSON line: MLUPS (million lattice-node updates per second) with a
+roofline block for the step kernel (algorithmic bytes per launch divided by
+the average launch time measured with HIP events on the solver's own stream)
+and a CPU baseline (the FP64 CPU oracle, a bounded sample of the same
+workload on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mpi-multi-process-version_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+WORKLOADS = {
+    "3d256": dict(dim=3, n=256, Ra=1e5, label="cavity3d_256_ra1e5_d3q19_d3q7"),
+    "2d1025": dict(dim=2, n=1025, Ra=1e6, label="cavity2d_1025_ra1e6_d2q9_d2q5"),
+}
+
+
+def traffic_from_profiles(workload):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by the copy-kernel calibration), if a summary for
+    this workload has been committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("bytes_per_launch")
+
+
+def cpu_baseline(cfg, threads, sample_steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lbm_oracle
+    lbm_oracle.set_threads(threads)
+    o = lbm_oracle.Oracle(cfg["dim"], cfg["n"], cfg["Ra"])
+    o.init_rest()
+    o.step(1)  # first touch of every array
+    t0 = time.perf_counter()
+    o.step(sample_steps)
+    dt = time.perf_counter() - t0
+    nodes = cfg["n"] ** cfg["dim"]
+    o.close()
+    return {"value": nodes * sample_steps / dt / 1e6, "unit": "MLUPS",
+            "cores": threads, "kind": "port",
+            "sample": f"{sample_steps} steps of the same {cfg['n']}^{cfg['dim']} cavity "
+                      f"(FP64 CPU oracle, {threads} OpenMP threads, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="3d256")
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="oracle steps for the CPU baseline (0: automatic)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs one cavity on one GPU; multi-GPU runs "
+              "(--gpus > 1) are not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import lbmhip
+    cfg = WORKLOADS[args.workload]
+    prec = lbmhip.PREC_FP64 if args.precision == "fp64" else lbmhip.PREC_FP32_SHIFTED
+    s = lbmhip.Solver(cfg["dim"], cfg["n"], cfg["Ra"], precision=prec, device=0)
+    s.init_rest()
+    if args.warmup:
+        s.step(args.warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    ms_events = s.time_steps(args.steps)   # HIP events on the solver's stream
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    nodes = cfg["n"] ** cfg["dim"]
+    sec = ms_events / 1e3
+    mlups = nodes * args.steps / sec / 1e6
+    bpu = s.bytes_per_update()
+    launch_s = sec / args.steps
+    achieved = bpu * nodes / launch_s / 1e9
+    eu, et = s.check(allow_diverged=True)
+    nu = s.nusselt()
+    s.close()
+
+    traffic = traffic_from_profiles(args.workload)
+    out = {
+        "metric": "MLUPS (million lattice-node updates per second), fused MRT-LBM step",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_events / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == lbmhip.PREC_FP32_SHIFTED else "f64",
+        "data": "synthetic: reference initial condition (rest, linear T)",
+        "config": {"workload": cfg["label"], "nodes": nodes, "grid": "erf A=3.2",
+                   "Ra": cfg["Ra"], "Pr": 0.71, "precision": args.precision,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "algorithmic_bytes_per_launch": bpu * nodes},
+        "wall_s": round(wall, 4),
+        "check": {"errU": eu, "errT": et, "nu_hot": nu[0], "nu_cold": nu[1]},
+    }
+    if not args.no_cpu_baseline:
+        steps = args.cpu_steps or (2 if cfg["dim"] == 3 else 40)
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_threads, steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * lbm_capi.h -- C ABI of the MI355X (gfx950) coupled MRT lattice-Boltzmann
+ * natural-convection solver.
+ *
+ * The reference (zzhao0930/MPI-Multi-Process-Version) has no plugin or FFI
+ * layer: its boundary is the sequence of argument-less subroutine calls in
+ * `program main` over the module-global state `commondata`
+ * (natural-convection_2dim_mono.f90:1-72 and :283-380;
+ *  natural-convection_3dim_mono.f90:1-77 and :347-443).  Each entry point
+ * below replaces one group of those calls; the citation is on each
+ * declaration.  A Fortran caller binds them with iso_c_binding (see
+ * mpi-multi-process-version_amd/fortran/lbm_capi_mod.f90).
+ *
+ * Conventions
+ *  - Every function returns an int32 status, LBM_OK (0) on success.  No C++
+ *    exception crosses the ABI.  lbm_last_error() gives a detail message for
+ *    the calling thread.
+ *  - The handle owns all device memory.  Host arrays belong to the caller:
+ *    contiguous FP64 arrays of nx*ny(*nz) values in Fortran order (i fastest,
+ *    then j, then k), copied synchronously.  w may be NULL in 2D.
+ *  - A handle is not thread-safe; each handle owns one HIP stream and
+ *    separate handles may run concurrently.  lbm_step() is asynchronous;
+ *    every getter synchronises the handle's stream.
+ *  - Axes follow the reference: 2D heated x-walls (hot at i=1) with gravity
+ *    along +y; 3D heated y-walls (hot at j=1) with gravity along +z.
+ */
+#ifndef LBM_CAPI_H
+#define LBM_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define LBM_OK                 0
+#define LBM_ERR_ARG            1  /* invalid argument or parameter set        */
+#define LBM_ERR_HIP            2  /* HIP runtime error (no device, launch...)  */
+#define LBM_ERR_DIVERGED       3  /* non-finite residual                      */
+#define LBM_ERR_NOT_CONVERGED  4  /* itc_max reached before epsU/epsT         */
+#define LBM_ERR_STATE          5  /* call out of order (e.g. step before init) */
+#define LBM_ERR_NOMEM          6  /* device or host allocation failed         */
+
+/* Grid maps.  The reference uses the erf map with constA = 3.2
+ * (natural-convection_2dim_mono.f90:85-91). */
+#define LBM_GRID_ERF   0  /* x_i = (erf(A(i/(N+1)-1/2))/erf(A/2)+1)/2        */
+#define LBM_GRID_TANH  1  /* same with tanh in place of erf (extension)      */
+
+#define LBM_PREC_FP32_SHIFTED  0  /* FP32 populations stored as f - w*rho0    */
+#define LBM_PREC_FP64          1  /* FP64 populations, the reference's form   */
+
+/* Inputs of the reference's `commondata` module, made run-time
+ * (natural-convection_2dim_mono.f90:6-35, constA at :85). */
+typedef struct lbm_params {
+    int32_t dim;          /* 2 or 3                                          */
+    int32_t nx, ny, nz;   /* nodes per axis; the cavity is square/cubic:
+                             nx == ny (== nz in 3D), nx >= 5               */
+    double  Ra, Pr, Ma;   /* Rayleigh, Prandtl, Mach numbers                 */
+    double  Thot, Tcold;  /* wall temperatures                               */
+    double  stretch;      /* grid stretching factor A (reference: 3.2)       */
+    int32_t grid_kind;    /* LBM_GRID_*                                      */
+    int32_t precision;    /* LBM_PREC_*                                      */
+    double  epsU, epsT;   /* convergence thresholds of lbm_run               */
+    int32_t check_every;  /* residual window in steps (reference: 2000)      */
+    int32_t device;       /* HIP device ordinal                              */
+    int64_t itc_max;      /* step limit of lbm_run                           */
+} lbm_params;
+
+typedef struct lbm_ctx *lbm_handle;
+
+/* Derived parameters returned by lbm_get_info(), in this order. */
+enum {
+    LBM_INFO_LLB = 0,  /* length_LB = 1/dx0           (2dim_mono.f90:110) */
+    LBM_INFO_DT,       /* dt = dx0*length_LB           (2dim_mono.f90:111) */
+    LBM_INFO_NU,       /* viscosity_LB                 (2dim_mono.f90:152) */
+    LBM_INFO_KAPPA,    /* thermal diffusivity          (2dim_mono.f90:153) */
+    LBM_INFO_GBETA,    /* g*beta                       (2dim_mono.f90:154) */
+    LBM_INFO_TAUF,     /* tau_f                        (2dim_mono.f90:157) */
+    LBM_INFO_SNU,      /* S_nu                         (2dim_mono.f90:160) */
+    LBM_INFO_SQ,       /* S_q                          (2dim_mono.f90:161) */
+    LBM_INFO_SIGK,     /* sigma_k (2dim_mono.f90:162, 3dim_mono.f90:229)   */
+    LBM_INFO_H,        /* cavity size in LB units = length_LB - 1          */
+    LBM_INFO_COUNT
+};
+
+/* mesh + the parameter part of initial (2dim_mono.f90:74-134, :151-174;
+ * 3dim_mono.f90:122-189, :206-229): validates the parameters (tau_f > 1/2,
+ * square/cubic grid), builds the grid and the interpolation tables and
+ * allocates the device state.  The populations are not set until
+ * lbm_init_rest() or lbm_init_fields(). */
+int32_t lbm_create(const lbm_params *params, lbm_handle *out);
+
+/* Frees the device state (the reference's deallocate block,
+ * 2dim_mono.f90:347-375). */
+int32_t lbm_destroy(lbm_handle h);
+
+/* Node coordinates in LB units, nx (ny, nz) values each (xGrid, y/yGrid, z
+ * after 2dim_mono.f90:113-116).  z may be NULL in 2D; any may be NULL. */
+int32_t lbm_get_grid(lbm_handle h, double *x, double *y, double *z);
+
+/* Derived parameters, LBM_INFO_COUNT doubles. */
+int32_t lbm_get_info(lbm_handle h, double *info);
+
+/* The initial condition of `initial` (2dim_mono.f90:199-240;
+ * 3dim_mono.f90:256-301): rho = 1, u = 0, T linear between the heated walls,
+ * equilibrium populations; residual snapshots cleared; itc = 0. */
+int32_t lbm_init_rest(lbm_handle h);
+
+/* Warm start / restart from macroscopic fields: rho = 1 and the given
+ * u, v, (w), T, equilibrium populations (the same formulas as
+ * 2dim_mono.f90:227-240); snapshots cleared; itc = 0.  v and w may be NULL
+ * (zero); T must not be NULL. */
+int32_t lbm_init_fields(lbm_handle h, const double *u, const double *v,
+                        const double *w, const double *T);
+
+/* nsteps iterations of the time-loop body: collision_U, collision_T,
+ * [update], interpolate, bounceback_u, bounceback_T, macro_u, macro_t
+ * (2dim_mono.f90:309-325; 3dim_mono.f90:375-391).  Asynchronous. */
+int32_t lbm_step(lbm_handle h, int64_t nsteps);
+
+/* check (2dim_mono.f90:755-805; 3dim_mono.f90:1027-1082): relative L2
+ * change of (u,v[,w]) and of T since the previous call, FP64; then the
+ * snapshot is updated.  The first call after initialisation returns 1.
+ * If a field and its change are both identically zero the ratio is
+ * reported as 0 (the reference would compute 0/0). */
+int32_t lbm_check(lbm_handle h, double *errU, double *errT);
+
+/* Mean Nusselt numbers of the hot and cold walls, FP64 (no reference
+ * counterpart; the reference only writes fields, 2dim_mono.f90:818-843):
+ * Nu = H dT/dn / (Thot - Tcold) from the one-sided quadratic through the
+ * wall value and the first two nodes, averaged with control-volume
+ * weights over the wall. */
+int32_t lbm_nusselt(lbm_handle h, double *nu_hot, double *nu_cold);
+
+/* gather_data (2dim_mono.f90:807-816; 3dim_mono.f90:1084-1095): the
+ * macroscopic fields of the current step, FP64, Fortran order.  Any pointer
+ * may be NULL to skip that field. */
+int32_t lbm_get_fields(lbm_handle h, double *rho, double *u, double *v,
+                       double *w, double *T);
+
+/* The time loop with its stopping rule (2dim_mono.f90:307-330): steps in
+ * windows of check_every, calling lbm_check at every multiple of
+ * check_every, until both residuals are <= epsU/epsT (LBM_OK) or
+ * itc >= itc_max (LBM_ERR_NOT_CONVERGED).  A non-finite residual returns
+ * LBM_ERR_DIVERGED (the reference would exit as if converged).  itc, errU
+ * and errT receive the final values. */
+int32_t lbm_run(lbm_handle h, int64_t *itc, double *errU, double *errT);
+
+/* Iteration counter (the reference's itc, 2dim_mono.f90:30). */
+int32_t lbm_get_itc(lbm_handle h, int64_t *itc);
+
+/* Runs nsteps steps like lbm_step and returns the elapsed time between two
+ * HIP events recorded on the handle's stream around them, in ms
+ * (the reference times with MPI_WTIME, 2dim_mono.f90:300-337). */
+int32_t lbm_time_steps(lbm_handle h, int64_t nsteps, double *ms);
+
+/* Blocks until the handle's stream is idle. */
+int32_t lbm_synchronize(lbm_handle h);
+
+/* Exact checkpoint of the whole solver state (populations, macroscopic
+ * fields, residual snapshots, itc).  No reference counterpart (the
+ * reference has no restart).  lbm_state_size gives the byte count. */
+int32_t lbm_state_size(lbm_handle h, int64_t *bytes);
+int32_t lbm_save_state(lbm_handle h, void *buf, int64_t bytes);
+int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes);
+
+/* Bytes moved per lattice update by one step kernel, as counted for the
+ * roofline: every population read once and written once. */
+int32_t lbm_bytes_per_update(lbm_handle h, double *bytes);
+
+const char *lbm_status_string(int32_t status);
+const char *lbm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,160 @@
+// lbm_aux.hip -- FP64 helper kernels: the reference initial condition, the
+// windowed residual (check) and the Nusselt reduction.  Reductions are
+// deterministic: a fixed grid accumulates per-thread partial sums, reduces
+// them per block with an LDS tree, and one block adds the block results in
+// a fixed order.  Compiled with -ffp-contract=off.
+#include "lbm_internal.h"
+
+namespace lbm {
+
+// initial (2dim_mono.f90:199-212, 3dim_mono.f90:256-271): rho = 1, u = 0,
+// T linear across the heated direction (x in 2D, y in 3D).
+__global__ void k_init_rest(double *rho, double *u, double *v, double *w, double *T,
+                            int n, int dim, long nn, double Thot, double Tcold) {
+    long node = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nn) return;
+    int ih = dim == 2 ? (int)(node % n) : (int)((node / n) % n);
+    rho[node] = 1.0;
+    u[node] = 0.0;
+    v[node] = 0.0;
+    if (dim == 3) w[node] = 0.0;
+    T[node] = (double)ih / (double)(n - 1) * (Tcold - Thot) + Thot;
+}
+
+template <int NQ>
+__device__ void block_reduce_store(double (&acc)[NQ], double *out) {
+    __shared__ double sh[NQ][kRedThreads];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) sh[q][t] = acc[q];
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int q = 0; q < NQ; q++) sh[q][t] += sh[q][t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) out[q] = sh[q][0];
+    }
+}
+
+// check (2dim_mono.f90:771-798, 3dim_mono.f90:1043-1070): partial sums of
+// |u - u_prev|^2, |u|^2, (T - T_prev)^2, T^2; the snapshot is updated.
+template <int DIM>
+__global__ __launch_bounds__(kRedThreads) void k_resid_part(
+    const double *__restrict__ u, const double *__restrict__ v,
+    const double *__restrict__ w, const double *__restrict__ T,
+    double *__restrict__ up, double *__restrict__ vp, double *__restrict__ wp,
+    double *__restrict__ Tp, long nn, double *__restrict__ part) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
+         i += (long)gridDim.x * blockDim.x) {
+        double uu = u[i], vv = v[i];
+        double du = uu - up[i], dv = vv - vp[i];
+        acc[0] += du * du + dv * dv;
+        acc[1] += uu * uu + vv * vv;
+        if constexpr (DIM == 3) {
+            double ww = w[i], dw = ww - wp[i];
+            acc[0] += dw * dw;
+            acc[1] += ww * ww;
+            wp[i] = ww;
+        }
+        double tt = T[i], dT = tt - Tp[i];
+        acc[2] += dT * dT;
+        acc[3] += tt * tt;
+        up[i] = uu;
+        vp[i] = vv;
+        Tp[i] = tt;
+    }
+    block_reduce_store<4>(acc, part + 4 * blockIdx.x);
+}
+
+// Mean Nusselt numbers of the hot and cold walls: one-sided quadratic
+// gradient through (0, T_wall), (d1, T1), (d2, T2), control-volume weights.
+__device__ inline double wall_gradient(double Tw, double d1, double T1, double d2,
+                                       double T2) {
+    return Tw * (-(d1 + d2) / (d1 * d2)) + T1 * d2 / (d1 * (d2 - d1)) -
+           T2 * d1 / (d2 * (d2 - d1));
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_nu_part(
+    const double *__restrict__ T, const double *__restrict__ x,
+    const double *__restrict__ cvw, int n, int dim, double H, double Thot,
+    double Tcold, double *__restrict__ part) {
+    double acc[2] = {0.0, 0.0};
+    const long nw = dim == 2 ? (long)n : (long)n * n;
+    const double dT = Thot - Tcold;
+    const double d1h = x[0], d2h = x[1], d1c = H - x[n - 1], d2c = H - x[n - 2];
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nw;
+         t += (long)gridDim.x * blockDim.x) {
+        long h1, h2, c1, c2;
+        double wgt;
+        if (dim == 2) {  // heated x-walls; wall coordinate j = t
+            h1 = (long)n * t;
+            h2 = 1 + (long)n * t;
+            c1 = (n - 1) + (long)n * t;
+            c2 = (n - 2) + (long)n * t;
+            wgt = cvw[t];
+        } else {  // heated y-walls; wall coordinates (i, k)
+            long i = t % n, k = t / n;
+            h1 = i + (long)n * ((long)n * k);
+            h2 = i + (long)n * (1 + (long)n * k);
+            c1 = i + (long)n * ((n - 1) + (long)n * k);
+            c2 = i + (long)n * ((n - 2) + (long)n * k);
+            wgt = cvw[i] * cvw[k];
+        }
+        double gh = wall_gradient(Thot, d1h, T[h1], d2h, T[h2]);
+        double gc = wall_gradient(Tcold, d1c, T[c1], d2c, T[c2]);
+        acc[0] += -H * gh / dT * wgt;
+        acc[1] += H * gc / dT * wgt;
+    }
+    block_reduce_store<2>(acc, part + 2 * blockIdx.x);
+}
+
+// Adds nparts partial results of NQ values each, in a fixed order.
+template <int NQ>
+__global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double *__restrict__ part,
+                                                              int nparts, double *__restrict__ out) {
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) acc[q] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) acc[q] += part[NQ * b + q];
+    }
+    block_reduce_store<NQ>(acc, out);
+}
+
+hipError_t launch_init_rest(const Ctx &c) {
+    dim3 block(256), grid((unsigned)((c.nn + 255) / 256));
+    hipLaunchKernelGGL(k_init_rest, grid, block, 0, c.stream, c.rho, c.u, c.v, c.w, c.T,
+                       c.n, c.dim, c.nn, c.p.Thot, c.p.Tcold);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const Ctx &c) {
+    dim3 block(kRedThreads), grid(c.red_blocks);
+    if (c.dim == 2)
+        hipLaunchKernelGGL(k_resid_part<2>, grid, block, 0, c.stream, c.u, c.v, c.w, c.T,
+                           c.up, c.vp, c.wp, c.Tp, c.nn, c.d_part);
+    else
+        hipLaunchKernelGGL(k_resid_part<3>, grid, block, 0, c.stream, c.u, c.v, c.w, c.T,
+                           c.up, c.vp, c.wp, c.Tp, c.nn, c.d_part);
+    hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), block, 0, c.stream, c.d_part,
+                       c.red_blocks, c.d_red);
+    return hipGetLastError();
+}
+
+hipError_t launch_nusselt(const Ctx &c) {
+    dim3 block(kRedThreads), grid(c.red_blocks);
+    hipLaunchKernelGGL(k_nu_part, grid, block, 0, c.stream, c.T, c.d_x, c.d_cvw, c.n, c.dim,
+                       c.info[LBM_INFO_H], c.p.Thot, c.p.Tcold, c.d_part);
+    hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), block, 0, c.stream, c.d_part,
+                       c.red_blocks, c.d_red + 4);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

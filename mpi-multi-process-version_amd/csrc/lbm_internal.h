@@ -1,0 +1,70 @@
+// lbm_internal.h -- state behind an lbm_handle and the launchers shared by
+// the translation units of liblbm_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_capi.h"
+#include "lbm_step.h"
+
+namespace lbm {
+
+// Device-side state of one cavity.  Populations live in two ping-pong
+// buffers of (Qf + Qg) SoA planes each; the FP64 macroscopic fields are
+// written on the last step of every lbm_step call.
+struct Ctx {
+    lbm_params p{};
+    int dim = 2, n = 0, qf = 9, qg = 5;
+    long nn = 0, plane = 0;
+    bool fp64 = false;
+    double info[LBM_INFO_COUNT] = {};
+    double dx0 = 0;
+    std::vector<double> x;      // node coordinates (LB units), same on every axis
+    std::vector<double> tref;   // conduction profile at each heated-axis index
+    std::vector<int> st;        // 3-point stencils, 3 per grid index
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *pbuf[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *T = nullptr;
+    double *up = nullptr, *vp = nullptr, *wp = nullptr, *Tp = nullptr;
+    double *force = nullptr;    // FP64 path only
+    int2 *d_off = nullptr;
+    void *d_wt = nullptr;       // weights in the storage type
+    double *d_x = nullptr;      // node coordinates
+    double *d_cvw = nullptr;    // control-volume widths / H
+    float *d_tref = nullptr;    // FP32 conduction profile (shifted storage)
+    double *d_part = nullptr;   // reduction partials
+    double *d_red = nullptr;    // reduction results
+    int red_blocks = 0;
+    int64_t itc = 0;
+    bool initialized = false;
+    StepArgs<float> a32{};
+    StepArgs<double> a64{};
+};
+
+// Fills the per-launch constants of both argument blocks (pointers to the
+// current buffers are set by the caller).
+void fill_args(Ctx &c);
+
+// Launchers (lbm_step_f32.hip / lbm_step_f64.hip).
+hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a, hipStream_t s);
+hipError_t launch_prologue_f32(int dim, const StepArgs<float> &a,
+                               const StepArgs<double> &ad, hipStream_t s);
+hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a, hipStream_t s);
+hipError_t launch_prologue_f64(int dim, const StepArgs<double> &a,
+                               const StepArgs<double> &ad, hipStream_t s);
+
+// Auxiliary kernels (lbm_aux.hip), all FP64.
+hipError_t launch_init_rest(const Ctx &c);
+hipError_t launch_residual(const Ctx &c);   // results: d_red[0..3]
+hipError_t launch_nusselt(const Ctx &c);    // results: d_red[4..5]
+
+constexpr int kRedThreads = 256;
+constexpr int kRedMaxBlocks = 1024;
+
+}  // namespace lbm

@@ -1,0 +1,577 @@
+// lbm_step.h -- fused pull kernel of one time step of the coupled solver,
+// templated on dimension, storage type and population shift.  Included by
+// lbm_step_f32.hip (FP32 shifted populations, the production path) and
+// lbm_step_f64.hip (FP64 populations in the reference's form, compiled
+// without FMA contraction).
+//
+// One launch of k_step maps the reference's loop body (collision_U,
+// collision_T, interpolate, bounceback_u, bounceback_T, macro_u, macro_t;
+// natural-convection_2dim_mono.f90:311-325, 3dim_mono.f90:377-391) onto one
+// pull sweep, rotated so that a launch starts where the populations are
+// written: it reads the post-collision populations P(k) of every node's
+// 3-point neighbourhood, and per node
+//   1. interpolates them onto the node (2dim_mono.f90:591-646,
+//      3dim_mono.f90:751-862), giving f(k), g(k);
+//   2. applies the wall rules at wall nodes with the node's own P(k)
+//      (2dim_mono.f90:662-724, 3dim_mono.f90:878-986);
+//   3. forms rho(k), T(k) and u(k) = (sum c f(k) + F(k) dt/2)/rho(k) with
+//      F(k) = g beta T(k-1) rho(k-1) (2dim_mono.f90:726-753).  The FP32 path
+//      recovers rho(k-1) and T(k-1) as the zeroth moments of the node's own
+//      P(k), which the collision conserves, so the one-step force lag costs
+//      no storage; the FP64 path keeps F per node like the reference;
+//   4. collides (MRT + Guo forcing in moment space, thermal MRT) with
+//      F(k+1) = g beta T(k) rho(k) (2dim_mono.f90:431-569) and writes P(k+1).
+// On the last launch of a batch the kernel also writes the FP64 macroscopic
+// fields that the residual check, the Nusselt number and the export read.
+//
+// Layout: populations are SoA planes, plane a at p + a*plane, flow planes
+// 0..Qf-1 then thermal planes Qf..Qf+Qg-1; node index i + n*j (+ n*n*k).
+// Shifted storage (SHIFT = true) keeps f - w_a and g - w_a*Tref(x), where
+// Tref is the linear conduction profile between the heated walls, so FP32
+// resolves the small deviations that carry the flow and the near-wall
+// temperature that sets the Nusselt number.  Quadratic interpolation streams
+// a linear profile exactly, so the shift costs one constant per thermal
+// direction along the heated axis.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "lbm_lattice.h"
+
+namespace lbm {
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// Per-launch constants.  Coefficients that depend only on the relaxation
+// rates are formed on the host in FP64, in the same operation order as the
+// oracle, and passed in.
+template <typename R> struct StepArgs {
+    const R *__restrict__ pin;
+    R *__restrict__ pout;
+    const int2 *__restrict__ off;  // per grid index: offsets of stencil points 0 and 2
+    const R *__restrict__ wt;      // per grid index: w(+1)[3], w(-1)[3]
+    double *__restrict__ rho;
+    double *__restrict__ u;
+    double *__restrict__ v;
+    double *__restrict__ w;
+    double *__restrict__ T;
+    R *__restrict__ force;  // FP64 path: per-node body force F(k), in place
+    long plane;
+    int n;
+    const R *__restrict__ tref;  // shifted path: Tref at each heated-axis index
+    R dt, half_dt, gbeta;
+    R tcorr;               // shifted path: -dt * dTref/dx (streaming of Tref)
+    R tw_hot, tw_cold;     // T_wall - Tref(wall node) (plain path: T_wall)
+    R s_nu, s_q, sig_k, q_e;
+    // Guo source coefficients (1 - s/2) times the lattice factor
+    R c_e, c_eps, c_q, c_pxx, c_pxy;     // 2D: 6h, -6h, -(h_q), 2h, h
+    R c3_e, c3_eps, c3_q, c3_9, c3_10, c3_11, c3_12, c3_14, c3_15;  // 3D
+};
+
+// ----------------------------------------------------------- collisions
+// Equilibrium moments with a selectable "base" density for the parts that
+// are linear in rho: base = rho (plain) or rho - 1 (shifted storage).
+template <int DIM, typename R>
+__device__ __forceinline__ void flow_meq(R base, R rho, R u, R v, R w, R *me) {
+    if constexpr (DIM == 2) {
+        R u2 = u * u + v * v;
+        me[0] = base;
+        me[1] = R(-2.0) * base + R(3.0) * rho * u2;
+        me[2] = base - R(3.0) * rho * u2;
+        me[3] = rho * u;
+        me[4] = -rho * u;
+        me[5] = rho * v;
+        me[6] = -rho * v;
+        me[7] = rho * (u * u - v * v);
+        me[8] = rho * (u * v);
+    } else {
+        R u2 = u * u + v * v + w * w;
+        R pxx = R(3.0) * u * u - u2, pww = v * v - w * w;
+        me[0] = base;
+        me[1] = R(-11.0) * base + R(19.0) * rho * u2;
+        me[2] = R(3.0) * base - R(5.5) * rho * u2;
+        me[3] = rho * u;
+        me[4] = R(-(2.0 / 3.0)) * rho * u;
+        me[5] = rho * v;
+        me[6] = R(-(2.0 / 3.0)) * rho * v;
+        me[7] = rho * w;
+        me[8] = R(-(2.0 / 3.0)) * rho * w;
+        me[9] = rho * pxx;
+        me[10] = R(-0.5) * rho * pxx;
+        me[11] = rho * pww;
+        me[12] = R(-0.5) * rho * pww;
+        me[13] = rho * u * v;
+        me[14] = rho * v * w;
+        me[15] = rho * u * w;
+        me[16] = R(0);
+        me[17] = R(0);
+        me[18] = R(0);
+    }
+}
+
+// Moment rows with a possibly non-zero collision increment: every relaxed
+// row, plus the momentum row along the body force (j_y in 2D, j_z in 3D).
+template <int DIM> constexpr bool flow_row_active(int r) {
+    if constexpr (DIM == 2) return r != 0 && r != 3;
+    else return r != 0 && r != 3 && r != 5;
+}
+
+// m = M f, relax towards m_eq, add (I - S/2) M F_Guo dt, f_post = M^-1 m.
+// The body force is along +y (2D) or +z (3D) with magnitude Fb.
+template <int DIM, typename R, bool SHIFT>
+__device__ __forceinline__ void collide_flow(const R *f, R base, R rho, R u, R v,
+                                             R w, R Fb, const StepArgs<R> &a,
+                                             R *fp) {
+    using FL = Flow<DIM>;
+    constexpr int Q = FL::Q;
+    R m[Q], me[Q], src[Q];
+    static_for<0, Q>([&](auto Rr) {
+        constexpr int r = decltype(Rr)::value;
+        R acc = R(0);
+        static_for<0, Q>([&](auto Aa) {
+            constexpr int al = decltype(Aa)::value;
+            constexpr double co = FL::row(r, al);
+            if constexpr (co == 1.0) acc += f[al];
+            else if constexpr (co == -1.0) acc -= f[al];
+            else if constexpr (co != 0.0) acc += R(co) * f[al];
+        });
+        m[r] = acc;
+    });
+    flow_meq<DIM, R>(base, rho, u, v, w, me);
+    R s[Q];
+    if constexpr (DIM == 2) {
+        // S = diag(0, Snu, Snu, 0, Sq, 0, Sq, Snu, Snu)  (2dim_mono.f90:465-473)
+        const R sn = a.s_nu, sq = a.s_q;
+        s[0] = 0; s[1] = sn; s[2] = sn; s[3] = 0; s[4] = sq;
+        s[5] = 0; s[6] = sq; s[7] = sn; s[8] = sn;
+        // Guo source with F = (0, Fb)  (2dim_mono.f90:478-486)
+        R uF = v * Fb;
+        src[0] = 0;
+        src[1] = a.c_e * uF;
+        src[2] = a.c_eps * uF;
+        src[3] = 0;
+        src[4] = 0;
+        src[5] = Fb;
+        src[6] = a.c_q * Fb;
+        src[7] = a.c_pxx * (-(v * Fb));
+        src[8] = a.c_pxy * (u * Fb);
+    } else {
+        // 3dim_mono.f90:542-560
+        const R sn = a.s_nu, sq = a.s_q;
+        s[0] = 0; s[1] = sn; s[2] = sn; s[3] = 0; s[4] = sq; s[5] = 0;
+        s[6] = sq; s[7] = 0; s[8] = sq;
+        s[9] = sn; s[10] = sn; s[11] = sn; s[12] = sn; s[13] = sn; s[14] = sn;
+        s[15] = sn; s[16] = sq; s[17] = sq; s[18] = sq;
+        // Guo source with F = (0, 0, Fb)  (3dim_mono.f90:563-583)
+        R uF = w * Fb, am = -(w * Fb);
+        src[0] = 0; src[1] = a.c3_e * uF; src[2] = a.c3_eps * uF;
+        src[3] = 0; src[4] = 0; src[5] = 0; src[6] = 0;
+        src[7] = Fb; src[8] = a.c3_q * Fb;
+        src[9] = a.c3_9 * am; src[10] = a.c3_10 * am;
+        src[11] = a.c3_11 * am; src[12] = a.c3_12 * am;
+        src[13] = 0; src[14] = a.c3_14 * (v * Fb); src[15] = a.c3_15 * (u * Fb);
+        src[16] = 0; src[17] = 0; src[18] = 0;
+    }
+    if constexpr (std::is_same<R, double>::value) {
+        // FP64: m_post = m - S(m - m_eq) + src dt, f_post = M^-1 m_post, in
+        // the operation order of the FP64 oracle.
+        static_for<0, Q>([&](auto Rr) {
+            constexpr int r = decltype(Rr)::value;
+            R mp = m[r] - s[r] * (m[r] - me[r]) + src[r] * a.dt;
+            m[r] = mp / flow_norm<DIM>(r);
+        });
+        static_for<0, Q>([&](auto Aa) {
+            constexpr int al = decltype(Aa)::value;
+            R acc = R(0);
+            static_for<0, Q>([&](auto Rr) {
+                constexpr int r = decltype(Rr)::value;
+                constexpr double co = FL::row(r, al);
+                if constexpr (co == 1.0) acc += m[r];
+                else if constexpr (co == -1.0) acc -= m[r];
+                else if constexpr (co != 0.0) acc += R(co) * m[r];
+            });
+            fp[al] = acc;
+        });
+    } else {
+        // FP32: the same update written as an increment,
+        // f_post = f + M^-1 (-S(m - m_eq) + src dt).  The conserved moments
+        // have a zero increment, so mass and momentum pass through exactly
+        // instead of being scaled every step by the rounded 1/|row|^2.
+        static_for<0, Q>([&](auto Rr) {
+            constexpr int r = decltype(Rr)::value;
+            if constexpr (flow_row_active<DIM>(r))
+                m[r] = (src[r] * a.dt - s[r] * (m[r] - me[r])) * R(1.0 / flow_norm<DIM>(r));
+        });
+        static_for<0, Q>([&](auto Aa) {
+            constexpr int al = decltype(Aa)::value;
+            R acc = R(0);
+            static_for<0, Q>([&](auto Rr) {
+                constexpr int r = decltype(Rr)::value;
+                constexpr double co = FL::row(r, al);
+                if constexpr (flow_row_active<DIM>(r)) {
+                    if constexpr (co == 1.0) acc += m[r];
+                    else if constexpr (co == -1.0) acc -= m[r];
+                    else if constexpr (co != 0.0) acc += R(co) * m[r];
+                }
+            });
+            fp[al] = f[al] + acc;
+        });
+    }
+}
+
+// n = N g, relax towards n_eq, g_post = N^-1 n.  baseT = T (plain) or
+// T - Tref (shifted).  (2dim_mono.f90:522-569, 3dim_mono.f90:671-730)
+template <int DIM, typename R>
+__device__ __forceinline__ void collide_heat(const R *g, R baseT, R T, R u, R v,
+                                             R w, const StepArgs<R> &a, R *gp) {
+    using HT = Heat<DIM>;
+    constexpr int Q = HT::Q;
+    R nm[Q], ne[Q], q[Q];
+    static_for<0, Q>([&](auto Rr) {
+        constexpr int r = decltype(Rr)::value;
+        R acc = R(0);
+        static_for<0, Q>([&](auto Aa) {
+            constexpr int al = decltype(Aa)::value;
+            constexpr double co = HT::row(r, al);
+            if constexpr (co == 1.0) acc += g[al];
+            else if constexpr (co == -1.0) acc -= g[al];
+            else if constexpr (co != 0.0) acc += R(co) * g[al];
+        });
+        nm[r] = acc;
+    });
+    ne[0] = baseT;
+    ne[1] = T * u;
+    ne[2] = T * v;
+    q[0] = R(1.0);
+    q[1] = a.sig_k;
+    q[2] = a.sig_k;
+    if constexpr (DIM == 2) {
+        ne[3] = baseT * R(0.5);
+        ne[4] = R(0);
+        q[3] = a.q_e;
+        q[4] = a.q_e;
+    } else {
+        ne[3] = T * w;
+        ne[4] = baseT * R(0.5);
+        ne[5] = R(0);
+        ne[6] = R(0);
+        q[3] = a.sig_k;
+        q[4] = a.q_e;
+        q[5] = a.q_e;
+        q[6] = a.q_e;
+    }
+    if constexpr (std::is_same<R, double>::value) {
+        static_for<0, Q>([&](auto Rr) {
+            constexpr int r = decltype(Rr)::value;
+            nm[r] = nm[r] - q[r] * (nm[r] - ne[r]);
+        });
+        static_for<0, Q>([&](auto Aa) {
+            constexpr int al = decltype(Aa)::value;
+            R acc = R(0);
+            static_for<0, Q>([&](auto Rr) {
+                constexpr int r = decltype(Rr)::value;
+                constexpr int co = HT::ninv[al][r];
+                if constexpr (co == 1) acc += nm[r];
+                else if constexpr (co == -1) acc -= nm[r];
+                else if constexpr (co != 0) acc += R(co) * nm[r];
+            });
+            gp[al] = acc / double(HT::den);
+        });
+    } else {
+        // increment form; row 0 (temperature) is conserved exactly because
+        // n_0 and n_eq,0 are the same sum
+        static_for<1, Q>([&](auto Rr) {
+            constexpr int r = decltype(Rr)::value;
+            nm[r] = -q[r] * (nm[r] - ne[r]);
+        });
+        static_for<0, Q>([&](auto Aa) {
+            constexpr int al = decltype(Aa)::value;
+            R acc = R(0);
+            static_for<1, Q>([&](auto Rr) {
+                constexpr int r = decltype(Rr)::value;
+                constexpr int co = HT::ninv[al][r];
+                if constexpr (co == 1) acc += nm[r];
+                else if constexpr (co == -1) acc -= nm[r];
+                else if constexpr (co != 0) acc += R(co) * nm[r];
+            });
+            gp[al] = g[al] + acc * R(1.0 / HT::den);
+        });
+    }
+}
+
+// ------------------------------------------------------- interpolation
+// Quadratic Lagrange weights are precomputed per (grid index, sign of the
+// lattice velocity) in the reference's formula (2dim_mono.f90:649-660);
+// a zero velocity component gives the weights (0, 1, 0), so the reference's
+// trivial interpolation stages reduce exactly to copies and are skipped.
+template <typename R> struct AxisStencil {
+    int o0, o2;     // offsets of stencil points 0 and 2 (point 1 is the node)
+    R wp[3], wm[3]; // weights for c = +1 and c = -1
+};
+
+template <typename R>
+__device__ __forceinline__ AxisStencil<R> load_stencil(const StepArgs<R> &a, int idx) {
+    AxisStencil<R> s;
+    int2 o = a.off[idx];
+    s.o0 = o.x;
+    s.o2 = o.y;
+    const R *wt = a.wt + 6 * idx;
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        s.wp[b] = wt[b];
+        s.wm[b] = wt[3 + b];
+    }
+    return s;
+}
+
+// FP64: the reference's sum w0 f0 + w1 f1 + w2 f2.  FP32: the same value as
+// f1 + w0 (f0 - f1) + w2 (f2 - f1) (the weights sum to one), so that the
+// rounding of the stored weights acts on differences between neighbours
+// only and a uniform field streams unchanged.
+template <typename R>
+__device__ __forceinline__ R lerp3(const R *w, R f0, R f1, R f2) {
+    if constexpr (std::is_same<R, double>::value) return w[0] * f0 + w[1] * f1 + w[2] * f2;
+    else return f1 + w[0] * (f0 - f1) + w[2] * (f2 - f1);
+}
+
+// One population: a 1D interpolation when one velocity component is
+// non-zero, a two-stage tensor interpolation (first along the higher axis,
+// then along the lower one) when two are.
+template <int CX, int CY, int CZ, typename R>
+__device__ __forceinline__ R interp_pop(const R *__restrict__ F, long node, R fc,
+                                        const AxisStencil<R> *st, const long *stride) {
+    constexpr int c[3] = {CX, CY, CZ};
+    constexpr int nz = (CX != 0) + (CY != 0) + (CZ != 0);
+    if constexpr (nz == 1) {
+        constexpr int ax = CX ? 0 : (CY ? 1 : 2);
+        const AxisStencil<R> &s = st[ax];
+        const R *w = c[ax] > 0 ? s.wp : s.wm;
+        return lerp3(w, F[node + s.o0 * stride[ax]], fc, F[node + s.o2 * stride[ax]]);
+    } else {
+        constexpr int q = CX ? 0 : 1;          // second (outer) stage axis
+        constexpr int p = CZ ? 2 : 1;          // first (inner) stage axis
+        const AxisStencil<R> &sp = st[p], &sq = st[q];
+        const R *wpp = c[p] > 0 ? sp.wp : sp.wm;
+        const R *wqq = c[q] > 0 ? sq.wp : sq.wm;
+        const long dp0 = sp.o0 * stride[p], dp2 = sp.o2 * stride[p];
+        const long n0 = node + sq.o0 * stride[q], n2 = node + sq.o2 * stride[q];
+        R c0 = lerp3(wpp, F[n0 + dp0], F[n0], F[n0 + dp2]);
+        R c1 = lerp3(wpp, F[node + dp0], fc, F[node + dp2]);
+        R c2 = lerp3(wpp, F[n2 + dp0], F[n2], F[n2 + dp2]);
+        return lerp3(wqq, c0, c1, c2);
+    }
+}
+
+// ---------------------------------------------------------------- step
+template <int DIM, typename R, bool SHIFT, bool MACRO>
+__global__ __launch_bounds__(256) void k_step(StepArgs<R> a) {
+    using FL = Flow<DIM>;
+    using HT = Heat<DIM>;
+    constexpr int QF = FL::Q, QG = HT::Q;
+    const int n = a.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = DIM == 3 ? (int)blockIdx.z : 0;
+    if (i >= n || j >= n) return;
+    const long sy = n, sz = (long)n * n;
+    const long node = i + sy * j + sz * k;
+    const int idx[3] = {i, j, k};
+    const long stride[3] = {1, sy, sz};
+    AxisStencil<R> st[3];
+    st[0] = load_stencil(a, i);
+    st[1] = load_stencil(a, j);
+    if constexpr (DIM == 3) st[2] = load_stencil(a, k);
+
+    // own-node post-collision values (the stencils' centre points)
+    R pcf[QF], pcg[QG], f[QF], g[QG];
+    static_for<0, QF>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        pcf[al] = a.pin[al * a.plane + node];
+    });
+    static_for<0, QG>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        pcg[al] = a.pin[(QF + al) * a.plane + node];
+    });
+
+    constexpr int HAX = heated_axis<DIM>();
+    const R tloc = SHIFT ? a.tref[idx[HAX]] : R(0);
+
+    // 1. interpolation-supplemented streaming
+    f[0] = pcf[0];
+    g[0] = pcg[0];
+    static_for<1, QF>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        f[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2]>(
+            a.pin + al * a.plane, node, pcf[al], st, stride);
+    });
+    static_for<1, QG>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        g[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2]>(
+            a.pin + (QF + al) * a.plane, node, pcg[al], st, stride);
+        if constexpr (SHIFT && FL::c[al][HAX] != 0) {
+            constexpr double wc = HT::w[al] * FL::c[al][HAX];
+            g[al] += R(wc) * a.tcorr;
+        }
+    });
+
+    // 2. walls: populations entering the fluid take the value of their
+    // opposite at the same node; heated walls use anti-bounce-back.
+    static_for<0, DIM>([&](auto AX) {
+        constexpr int ax = decltype(AX)::value;
+        const int side = idx[ax] == 0 ? 1 : (idx[ax] == n - 1 ? -1 : 0);
+        if (side != 0) {
+            static_for<1, QF>([&](auto A) {
+                constexpr int al = decltype(A)::value;
+                constexpr int ca = FL::c[al][ax];
+                if constexpr (ca != 0) {
+                    if (ca * side > 0) f[al] = pcf[opposite<DIM>(al)];
+                }
+            });
+            static_for<1, QG>([&](auto A) {
+                constexpr int al = decltype(A)::value;
+                constexpr int ca = FL::c[al][ax];
+                if constexpr (ca != 0) {
+                    if (ca * side > 0) {
+                        R gpo = pcg[opposite<DIM>(al)];
+                        if constexpr (ax == heated_axis<DIM>()) {
+                            constexpr double w2 = 2.0 * HT::w[al];
+                            R tw = side > 0 ? a.tw_hot : a.tw_cold;
+                            g[al] = -gpo + R(w2) * tw;
+                        } else {
+                            g[al] = gpo;
+                        }
+                    }
+                }
+            });
+        }
+    });
+
+    // 3. macroscopic fields of this step and the force of the previous one.
+    // The FP64 path keeps F(k) per node, as the reference does (Fx/Fy/Fz,
+    // 2dim_mono.f90:47); the FP32 path recovers it from the conserved
+    // zeroth moments of the node's own P(k).
+    R F_prev;
+    if constexpr (std::is_same<R, double>::value) {
+        F_prev = a.force[node];
+    } else {
+        R sp_f = R(0), sp_g = R(0);
+        static_for<0, QF>([&](auto A) { sp_f += pcf[decltype(A)::value]; });
+        static_for<0, QG>([&](auto A) { sp_g += pcg[decltype(A)::value]; });
+        const R rho_prev = SHIFT ? R(1) + sp_f : sp_f;
+        const R T_prev = SHIFT ? tloc + sp_g : sp_g;
+        F_prev = a.gbeta * T_prev * rho_prev;
+    }
+
+    R srho = R(0), jx = R(0), jy = R(0), jz = R(0), sT = R(0);
+    static_for<0, QF>([&](auto A) { srho += f[decltype(A)::value]; });
+    static_for<1, QF>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        if constexpr (FL::c[al][0] > 0) jx += f[al];
+        if constexpr (FL::c[al][0] < 0) jx -= f[al];
+        if constexpr (FL::c[al][1] > 0) jy += f[al];
+        if constexpr (FL::c[al][1] < 0) jy -= f[al];
+        if constexpr (FL::c[al][2] > 0) jz += f[al];
+        if constexpr (FL::c[al][2] < 0) jz -= f[al];
+    });
+    static_for<0, QG>([&](auto A) { sT += g[decltype(A)::value]; });
+    const R rho = SHIFT ? R(1) + srho : srho;
+    const R T = SHIFT ? tloc + sT : sT;
+    R ux, uy, uz;
+    if constexpr (DIM == 2) {
+        ux = jx / rho;
+        uy = (jy + a.half_dt * F_prev) / rho;
+        uz = R(0);
+    } else {
+        ux = jx / rho;
+        uy = jy / rho;
+        uz = (jz + a.half_dt * F_prev) / rho;
+    }
+    if constexpr (MACRO) {
+        if constexpr (SHIFT) {
+            // FP64 sums of the stored deviations for the exported fields
+            double dr = 0.0, dT = 0.0;
+            static_for<0, QF>([&](auto A) { dr += double(f[decltype(A)::value]); });
+            static_for<0, QG>([&](auto A) { dT += double(g[decltype(A)::value]); });
+            a.rho[node] = 1.0 + dr;
+            a.T[node] = double(tloc) + dT;
+        } else {
+            a.rho[node] = double(rho);
+            a.T[node] = double(T);
+        }
+        a.u[node] = double(ux);
+        a.v[node] = double(uy);
+        if constexpr (DIM == 3) a.w[node] = double(uz);
+    }
+
+    // 4. collision with the force of this step
+    const R Fb = a.gbeta * T * rho;
+    if constexpr (std::is_same<R, double>::value) a.force[node] = Fb;
+    R fp[QF], gp[QG];
+    collide_flow<DIM, R, SHIFT>(f, SHIFT ? srho : rho, rho, ux, uy, uz, Fb, a, fp);
+    collide_heat<DIM, R>(g, SHIFT ? sT : T, T, ux, uy, uz, a, gp);
+    static_for<0, QF>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        a.pout[al * a.plane + node] = fp[al];
+    });
+    static_for<0, QG>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        a.pout[(QF + al) * a.plane + node] = gp[al];
+    });
+}
+
+// Initial collision (iteration 1's collision_U/collision_T): equilibrium
+// populations from the FP64 macroscopic fields (2dim_mono.f90:227-240),
+// collided in FP64 with F = g beta T rho, stored as R (shifted if SHIFT).
+template <int DIM, typename R, bool SHIFT>
+__global__ __launch_bounds__(256) void k_prologue(StepArgs<R> a, StepArgs<double> ad) {
+    using FL = Flow<DIM>;
+    using HT = Heat<DIM>;
+    constexpr int QF = FL::Q, QG = HT::Q;
+    const long nn = DIM == 2 ? (long)a.n * a.n : (long)a.n * a.n * a.n;
+    const long node = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nn) return;
+    const double rho = a.rho[node], u = a.u[node], v = a.v[node];
+    const double w = DIM == 3 ? a.w[node] : 0.0, T = a.T[node];
+    double f[QF], g[QG], fp[QF], gp[QG];
+    const double u2 = u * u + v * v + w * w;
+    static_for<0, QF>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        constexpr double wa = FL::w[al];
+        const double un = u * double(FL::c[al][0]) + v * double(FL::c[al][1]) +
+                          w * double(FL::c[al][2]);
+        f[al] = rho * wa * (1.0 + 3.0 * un + 4.5 * un * un - 1.5 * u2);
+    });
+    static_for<0, QG>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        constexpr double wa = HT::w[al];
+        const double un = u * double(FL::c[al][0]) + v * double(FL::c[al][1]) +
+                          w * double(FL::c[al][2]);
+        g[al] = wa * T * (1.0 + HT::cs2inv * un);
+    });
+    const double Fb = ad.gbeta * T * rho;
+    if constexpr (std::is_same<R, double>::value) a.force[node] = Fb;
+    collide_flow<DIM, double, false>(f, rho, rho, u, v, w, Fb, ad, fp);
+    collide_heat<DIM, double>(g, T, T, u, v, w, ad, gp);
+    static_for<0, QF>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        constexpr double wa = FL::w[al];
+        a.pout[al * a.plane + node] = R(SHIFT ? fp[al] - wa : fp[al]);
+    });
+    int hidx = DIM == 2 ? (int)(node % a.n) : (int)((node / a.n) % a.n);
+    const double tloc = SHIFT ? double(a.tref[hidx]) : 0.0;
+    static_for<0, QG>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        constexpr double wa = HT::w[al];
+        a.pout[(QF + al) * a.plane + node] = R(SHIFT ? gp[al] - wa * tloc : gp[al]);
+    });
+}
+
+}  // namespace lbm

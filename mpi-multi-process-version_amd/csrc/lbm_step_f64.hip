@@ -1,0 +1,32 @@
+// lbm_step_f64.hip -- FP64 instantiation of the fused step with unshifted
+// populations and the force stored per node.  This translation unit is
+// compiled with -ffp-contract=off, so every product and sum is rounded on
+// its own, in the same order as the FP64 CPU restatement.
+#include "lbm_internal.h"
+
+namespace lbm {
+
+hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a, hipStream_t s) {
+    dim3 block(64, 4, 1);
+    dim3 grid((a.n + 63) / 64, (a.n + 3) / 4, dim == 3 ? a.n : 1);
+    if (dim == 2) {
+        if (macro) hipLaunchKernelGGL((k_step<2, double, false, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_step<2, double, false, false>), grid, block, 0, s, a);
+    } else {
+        if (macro) hipLaunchKernelGGL((k_step<3, double, false, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_step<3, double, false, false>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_prologue_f64(int dim, const StepArgs<double> &a,
+                               const StepArgs<double> &ad, hipStream_t s) {
+    long nn = dim == 2 ? (long)a.n * a.n : (long)a.n * a.n * a.n;
+    dim3 block(256);
+    dim3 grid((unsigned)((nn + 255) / 256));
+    if (dim == 2) hipLaunchKernelGGL((k_prologue<2, double, false>), grid, block, 0, s, a, ad);
+    else hipLaunchKernelGGL((k_prologue<3, double, false>), grid, block, 0, s, a, ad);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

@@ -1,0 +1,213 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 CPU oracle.
+
+Tolerances are relative max-norm errors, max|gpu - ref| / scale, with the
+field's own maximum as scale for rho and T and the velocity scale
+max(|u|, |v|, |w|) for each velocity component:
+  * FP64 GPU path vs oracle: <= 1e-13 (the two compute the same operations
+    in the same order; the test also reports whether they are bit-exact);
+  * FP32 shifted GPU path vs FP64: <= 1e-5 on u, v, w, T and 1e-4 relative
+    on the mean Nusselt numbers.
+"""
+import numpy as np
+import pytest
+
+from helpers import field_errors, rel_max_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-13
+FP32_TOL = 1e-5
+NU_TOL = 1e-4
+
+
+def _pair(oracle_mod, hip, dim, n, Ra, precision, **kw):
+    o = oracle_mod.Oracle(dim, n, Ra, **{k: v for k, v in kw.items()
+                                         if k in ("Pr", "Ma", "Thot", "Tcold", "A")})
+    skw = {k: v for k, v in kw.items() if k in ("Pr", "Ma", "Thot", "Tcold")}
+    if "A" in kw:
+        skw["stretch"] = kw["A"]
+    s = hip.Solver(dim, n, Ra, precision=precision, **skw)
+    return o, s
+
+
+def _smooth_fields(dim, n, seed):
+    rng = np.random.default_rng(seed)
+    t = (np.arange(n) + 0.5) / n
+    if dim == 2:
+        Y, X = np.meshgrid(t, t, indexing="ij")
+        base = [np.sin(np.pi * X) * np.sin(np.pi * Y)]
+        amp = rng.uniform(0.5, 1.0, size=4)
+        u = 0.02 * amp[0] * np.sin(2 * np.pi * Y) * base[0]
+        v = -0.02 * amp[1] * np.sin(2 * np.pi * X) * base[0]
+        T = 1.0 - X + 0.05 * amp[2] * np.sin(3 * np.pi * X) * np.cos(np.pi * Y)
+        return u, v, None, T
+    Z, Y, X = np.meshgrid(t, t, t, indexing="ij")
+    amp = rng.uniform(0.5, 1.0, size=4)
+    b = np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)
+    u = 0.02 * amp[0] * np.sin(2 * np.pi * Z) * b
+    v = 0.015 * amp[1] * np.cos(np.pi * X) * b
+    w = -0.02 * amp[2] * np.sin(2 * np.pi * X) * b
+    T = 1.0 - Y + 0.05 * amp[3] * np.sin(3 * np.pi * Y) * np.cos(np.pi * Z)
+    return u, v, w, T
+
+
+@pytest.mark.parametrize("dim,n,Ra", [(2, 33, 1e4), (2, 65, 1e5), (2, 24, 1e6),
+                                      (3, 17, 1e5), (3, 12, 1e6)])
+def test_fp64_rest_start_matches_oracle(oracle_mod, hip, dim, n, Ra):
+    print(f"\n[fp64 rest] dim={dim} n={n} Ra={Ra}", flush=True)
+    o, s = _pair(oracle_mod, hip, dim, n, Ra, hip.PREC_FP64)
+    o.init_rest()
+    s.init_rest()
+    done = 0
+    for target in (1, 2, 10, 100):
+        o.step(target - done)
+        s.step(target - done)
+        done = target
+        errs = field_errors(s.fields(), o.fields())
+        exact = all(np.array_equal(a, b) for a, b in zip(s.fields(), o.fields()))
+        print(f"  itc={target} errs={errs} bit-exact={exact}", flush=True)
+        for k, e in errs.items():
+            assert e <= FP64_TOL, (target, k, e)
+    assert s.itc == o.itc == 100
+
+
+@pytest.mark.parametrize("dim,n,Ra,seed", [(2, 29, 1e5, 1), (3, 14, 1e5, 2)])
+def test_fp64_warm_start_matches_oracle(oracle_mod, hip, dim, n, Ra, seed):
+    print(f"\n[fp64 warm] dim={dim} n={n} Ra={Ra}", flush=True)
+    o, s = _pair(oracle_mod, hip, dim, n, Ra, hip.PREC_FP64)
+    u, v, w, T = _smooth_fields(dim, n, seed)
+    o.init_fields(u, v, w, T)
+    s.init_fields(u, v, w, T)
+    # the initial fields are returned unchanged before the first step
+    g0 = s.fields()
+    assert np.array_equal(g0[1], u) and np.array_equal(g0[4], T)
+    for nsteps in (1, 19):
+        o.step(nsteps)
+        s.step(nsteps)
+        errs = field_errors(s.fields(), o.fields())
+        print(f"  itc={o.itc} errs={errs}", flush=True)
+        for k, e in errs.items():
+            assert e <= FP64_TOL, (k, e)
+
+
+@pytest.mark.parametrize("dim,n,Ra,A", [(2, 33, 1e4, 3.2), (2, 41, 1e5, 1.5),
+                                        (3, 17, 1e5, 3.2)])
+def test_fp64_check_and_nusselt_match_oracle(oracle_mod, hip, dim, n, Ra, A):
+    print(f"\n[fp64 check/Nu] dim={dim} n={n} Ra={Ra} A={A}", flush=True)
+    o, s = _pair(oracle_mod, hip, dim, n, Ra, hip.PREC_FP64, A=A)
+    o.init_rest()
+    s.init_rest()
+    for k in range(2):
+        o.step(500)
+        s.step(500)
+        eo = o.check()
+        es = s.check()
+        print(f"  check {k}: oracle={eo} gpu={es}", flush=True)
+        if k == 0:
+            assert es == (1.0, 1.0)  # the snapshot starts at zero
+        assert abs(es[0] - eo[0]) <= 1e-12 * abs(eo[0])
+        assert abs(es[1] - eo[1]) <= 1e-12 * abs(eo[1])
+    nh, nc = s.nusselt()
+    oh, oc = o.nusselt()
+    print(f"  Nu gpu=({nh}, {nc}) oracle=({oh}, {oc})", flush=True)
+    assert abs(nh - oh) <= 1e-12 * abs(oh)
+    assert abs(nc - oc) <= 1e-12 * abs(oc)
+
+
+@pytest.mark.parametrize("dim,n,Ra,steps", [(2, 65, 1e5, 2000), (3, 21, 1e5, 500)])
+def test_fp32_shifted_matches_oracle(oracle_mod, hip, dim, n, Ra, steps):
+    print(f"\n[fp32 vs oracle] dim={dim} n={n} Ra={Ra} steps={steps}", flush=True)
+    o, s = _pair(oracle_mod, hip, dim, n, Ra, hip.PREC_FP32_SHIFTED)
+    o.init_rest()
+    s.init_rest()
+    o.step(steps)
+    s.step(steps)
+    errs = field_errors(s.fields(), o.fields())
+    print(f"  errs={errs}", flush=True)
+    names = ("u", "v", "w", "T") if dim == 3 else ("u", "v", "T")
+    for k in names:
+        assert errs[k] <= FP32_TOL, (k, errs[k])
+    nh, nc = s.nusselt()
+    oh, oc = o.nusselt()
+    print(f"  Nu gpu=({nh}, {nc}) oracle=({oh}, {oc})", flush=True)
+    assert abs(nh - oh) <= NU_TOL * abs(oh)
+    assert abs(nc - oc) <= NU_TOL * abs(oc)
+
+
+def test_run_stops_at_itc_max_and_checks(hip):
+    print("\n[run loop] 2D 33 fp32", flush=True)
+    s = hip.Solver(2, 33, 1e4, itc_max=5000, check_every=2000, epsU=1e-12, epsT=1e-12)
+    s.init_rest()
+    code, itc, eu, et = s.run()
+    assert code == hip.LBM_ERR_NOT_CONVERGED
+    assert itc == 5000
+    # the last check ran at itc = 4000; itc_max is not a check point
+    assert 0 < eu < 1 and 0 < et < 1
+
+
+def test_save_load_state_is_exact(hip):
+    print("\n[checkpoint] 2D 40 fp32", flush=True)
+    s = hip.Solver(2, 40, 1e5)
+    s.init_rest()
+    s.step(50)
+    buf = s.save_state()
+    s.step(50)
+    a = s.fields()
+    s.load_state(buf)
+    assert s.itc == 50
+    s.step(50)
+    b = s.fields()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_call_order_and_arguments(hip):
+    print("\n[errors]", flush=True)
+    s = hip.Solver(2, 17, 1e4)
+    with pytest.raises(hip.LbmError) as e:
+        s.step(1)
+    assert e.value.code == hip.LBM_ERR_STATE
+    with pytest.raises(hip.LbmError) as e:
+        hip.Solver(2, 17, 1e4, Ma=1e-9)  # tau_f -> 1/2
+    assert e.value.code == hip.LBM_ERR_ARG
+
+
+def test_large_2d_fp32_vs_fp64_gpu(hip):
+    """1025^2 at Ra = 1e6: the FP32 path after 2000 steps against the FP64
+    GPU path (itself checked against the oracle at small sizes above)."""
+    print("\n[2D 1025 fp32 vs fp64] 2000 steps", flush=True)
+    a = hip.Solver(2, 1025, 1e6, precision=hip.PREC_FP32_SHIFTED)
+    b = hip.Solver(2, 1025, 1e6, precision=hip.PREC_FP64)
+    a.init_rest()
+    b.init_rest()
+    a.step(2000)
+    b.step(2000)
+    errs = field_errors(a.fields(), b.fields())
+    print(f"  errs={errs}", flush=True)
+    for k in ("u", "v", "T"):
+        assert errs[k] <= FP32_TOL, (k, errs[k])
+    na, nb = a.nusselt(), b.nusselt()
+    print(f"  Nu fp32={na} fp64={nb}", flush=True)
+    assert abs(na[0] - nb[0]) <= NU_TOL * abs(nb[0])
+
+
+def test_large_3d_fp32_vs_fp64_gpu(hip):
+    """256^3 at Ra = 1e5 after 2000 steps (the reference's 3D itc_max)."""
+    print("\n[3D 256 fp32 vs fp64] 2000 steps", flush=True)
+    a = hip.Solver(3, 256, 1e5, precision=hip.PREC_FP32_SHIFTED)
+    a.init_rest()
+    a.step(2000)
+    fa = a.fields()
+    na = a.nusselt()
+    a.close()
+    b = hip.Solver(3, 256, 1e5, precision=hip.PREC_FP64)
+    b.init_rest()
+    b.step(2000)
+    fb = b.fields()
+    nb = b.nusselt()
+    b.close()
+    errs = field_errors(fa, fb)
+    print(f"  errs={errs} Nu fp32={na} fp64={nb}", flush=True)
+    for k in ("u", "v", "w", "T"):
+        assert errs[k] <= FP32_TOL, (k, errs[k])
+    assert abs(na[0] - nb[0]) <= NU_TOL * abs(nb[0])
