@@ -69,7 +69,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="oracle steps for the CPU baseline (0: automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (5 times), the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": a.numel() * 4, "launches": 5}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs one cavity on one GPU; multi-GPU runs "
               "(--gpus > 1) are not supported", file=sys.stderr)
@@ -82,6 +95,9 @@ def main():
     s.init_rest()
     if args.warmup:
         s.step(args.warmup)
+    if not args.no_autotune:
+        s.autotune(10)   # restores the state it started from
+    launch = s.get_launch()
     s.synchronize()
     t0 = time.perf_counter()
     ms_events = s.time_steps(args.steps)   # HIP events on the solver's stream
@@ -113,6 +129,7 @@ def main():
         "data": "synthetic: reference initial condition (rest, linear T)",
         "config": {"workload": cfg["label"], "nodes": nodes, "grid": "erf A=3.2",
                    "Ra": cfg["Ra"], "Pr": 0.71, "precision": args.precision,
+                   "launch": launch,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

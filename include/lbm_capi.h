@@ -165,6 +165,34 @@ int32_t lbm_state_size(lbm_handle h, int64_t *bytes);
 int32_t lbm_save_state(lbm_handle h, void *buf, int64_t bytes);
 int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes);
 
+/* Launch configuration of the step kernel (no reference counterpart: the
+ * reference's loops have no launch shape).
+ *  kind 0: one lattice node per thread, bx x by threads per workgroup.
+ *  kind 1 (3D, FP32 only): each workgroup owns a bx x by tile of columns and
+ *    marches zchunk planes along z with the plane tiles staged in LDS (shapes
+ *    64x4, 64x8, 128x2, 128x4, 32x8, 96x6; other shapes or unsupported edge
+ *    lengths run kind 0).
+ *  remap: tile order.  Workgroups are dealt round-robin to the 8 XCDs, each
+ *    with its own L2.  0 = x fastest, then y, then z; 1 = each XCD sweeps one
+ *    contiguous range of that order; 2 = each XCD owns a band of tile rows in
+ *    y and sweeps it plane by plane (needs the number of y tiles divisible by
+ *    8, else 0 is used).
+ *  occupancy (3D FP32 kind 0): 0, or 5 or 6 resident waves per SIMD, which
+ *    caps the registers per lane.
+ * bx must be a multiple of 32, bx*by a multiple of 64 and at most 1024. */
+typedef struct lbm_launch {
+    int32_t kind, bx, by, zchunk, remap, occupancy;
+} lbm_launch;
+int32_t lbm_set_launch(lbm_handle h, const lbm_launch *launch);
+/* The configuration that runs (kind 0 if kind 1 does not apply). */
+int32_t lbm_get_launch(lbm_handle h, lbm_launch *launch);
+
+/* Block-shape autotuner: times steps_per_trial steps for each candidate
+ * shape and tile order, keeps the fastest, and restores the solver state
+ * (populations, fields, itc) to what it was before the call.  best_ms gets
+ * the fastest time per step.  No reference counterpart. */
+int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms);
+
 /* Bytes moved per lattice update by one step kernel, as counted for the
  * roofline: every population read once and written once. */
 int32_t lbm_bytes_per_update(lbm_handle h, double *bytes);

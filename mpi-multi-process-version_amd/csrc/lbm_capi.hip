@@ -219,8 +219,8 @@ int32_t do_steps(Ctx &c, int64_t nsteps) {
     for (int64_t s = 0; s < nsteps; s++) {
         const bool macro = (s == nsteps - 1);
         bind_buffers(c);
-        hipError_t e = c.fp64 ? lbm::launch_step_f64(c.dim, macro, c.a64, c.stream)
-                              : lbm::launch_step_f32(c.dim, macro, c.a32, c.stream);
+        hipError_t e = c.fp64 ? lbm::launch_step_f64(c.dim, macro, c.a64, c.shape, c.stream)
+                              : lbm::launch_step_f32(c.dim, macro, c.a32, c.shape, c.stream);
         if (e != hipSuccess) return hip_fail(e, "step launch");
         c.cur = 1 - c.cur;
         c.itc++;
@@ -241,6 +241,13 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
         return fail(LBM_ERR_ARG, "grid must be square/cubic with nx >= 5");
     if (p.nx > 4096 || (p.dim == 3 && p.nx > 1024))
         return fail(LBM_ERR_ARG, "grid too large");
+    {
+        // kernels address a population plane with 32-bit byte offsets
+        const double esz = p.precision == LBM_PREC_FP64 ? 8.0 : 4.0;
+        const double nodes = p.dim == 2 ? (double)p.nx * p.nx : (double)p.nx * p.nx * p.nx;
+        if (nodes * esz >= 4294967296.0)
+            return fail(LBM_ERR_ARG, "grid too large for one population plane");
+    }
     if (!(p.Ra > 0) || !(p.Pr > 0) || !(p.Ma > 0) || !(p.stretch > 0) ||
         !std::isfinite(p.Ra) || !std::isfinite(p.Pr) || !std::isfinite(p.Ma) ||
         !std::isfinite(p.stretch) || !std::isfinite(p.Thot) || !std::isfinite(p.Tcold) ||
@@ -599,6 +606,111 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes) {
     }
     h->itc = hd.itc;
     h->initialized = true;
+    return LBM_OK;
+}
+
+int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
+    if (!h || !l) return fail(LBM_ERR_ARG, "null argument");
+    if ((l->kind != 0 && l->kind != 1) || l->bx < 32 || l->bx % 32 || l->by < 1 ||
+        l->bx * l->by > 1024 || (l->bx * l->by) % 64 || l->remap < 0 || l->remap > 2 ||
+        l->zchunk < 1 || (l->occupancy != 0 && l->occupancy != 5 && l->occupancy != 6))
+        return fail(LBM_ERR_ARG, "invalid launch configuration");
+    h->shape.kind = l->kind;
+    h->shape.bx = l->bx;
+    h->shape.by = l->by;
+    h->shape.zchunk = l->zchunk;
+    h->shape.remap = l->remap;
+    h->shape.occ = l->occupancy;
+    return LBM_OK;
+}
+
+int32_t lbm_get_launch(lbm_handle h, lbm_launch *l) {
+    if (!h || !l) return fail(LBM_ERR_ARG, "null argument");
+    const bool z = h->shape.kind == 1 && h->dim == 3 && !h->fp64 &&
+                   lbm::zmarch_supported(h->n, h->shape.bx, h->shape.by);
+    l->kind = z ? 1 : 0;
+    l->bx = h->shape.bx;
+    l->by = h->shape.by;
+    l->zchunk = h->shape.zchunk;
+    l->remap = h->shape.remap;
+    l->occupancy = (!z && h->dim == 3 && !h->fp64) ? h->shape.occ : 0;
+    return LBM_OK;
+}
+
+int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms) {
+    int32_t rc = check_handle(h, true);
+    if (rc) return rc;
+    if (steps_per_trial < 1) return fail(LBM_ERR_ARG, "steps_per_trial < 1");
+    // snapshot of everything the trial steps modify
+    std::vector<Segment> segs = state_segments(h);
+    std::vector<void *> saved(segs.size(), nullptr);
+    auto release = [&]() {
+        for (void *p : saved)
+            if (p) (void)hipFree(p);
+    };
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (size_t k = 0; k < segs.size(); k++) {
+        hipError_t e = hipMalloc(&saved[k], segs[k].bytes);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(saved[k], segs[k].dev, segs[k].bytes, hipMemcpyDeviceToDevice,
+                               h->stream);
+        if (e != hipSuccess) {
+            release();
+            return hip_fail(e, "autotune snapshot");
+        }
+    }
+    const int cur0 = h->cur;
+    const int64_t itc0 = h->itc;
+    const lbm::LaunchShape start = h->shape;
+    // candidates: (kind, bx, by, zchunk); each with both tile orders
+    static const int cand[][4] = {{0, 64, 1, 0},  {0, 64, 2, 0},  {0, 64, 4, 0},  {0, 128, 1, 0},
+                                  {0, 128, 2, 0}, {0, 128, 4, 0}, {0, 256, 1, 0}, {0, 256, 2, 0},
+                                  {0, 32, 8, 0},  {0, 64, 8, 0}};
+    lbm::LaunchShape best = start;
+    double best_t = 1e300;
+    for (const auto &cd : cand) {
+        const bool z = cd[0] == 1;
+        if (z && (h->dim != 3 || h->fp64 || !lbm::zmarch_supported(h->n, cd[1], cd[2]))) continue;
+        if (!z && cd[1] > 2 * h->n && cd[1] > 64) continue;
+        const int nocc = (h->dim == 3 && !h->fp64 && !z) ? 3 : 1;
+        for (int combo = 0; combo < 3 * nocc; combo++) {
+            const int remap = combo % 3;
+            static const int occs[3] = {0, 5, 6};
+            h->shape.kind = cd[0];
+            h->shape.bx = cd[1];
+            h->shape.by = cd[2];
+            h->shape.zchunk = z ? cd[3] : 32;
+            h->shape.remap = remap;
+            h->shape.occ = occs[combo / 3];
+            double ms = 0;
+            // one untimed warm-up launch per candidate, then the timed trial
+            if ((rc = do_steps(*h, 1)) || (rc = lbm_time_steps(h, steps_per_trial, &ms))) {
+                release();
+                return rc;
+            }
+            ms /= steps_per_trial;
+            if (ms < best_t) {
+                best_t = ms;
+                best = h->shape;
+            }
+        }
+    }
+    h->shape = best;
+    // restore: the snapshot was taken from buffer cur0
+    h->cur = cur0;
+    segs = state_segments(h);
+    for (size_t k = 0; k < segs.size(); k++) {
+        hipError_t e = hipMemcpyAsync(segs[k].dev, saved[k], segs[k].bytes,
+                                      hipMemcpyDeviceToDevice, h->stream);
+        if (e != hipSuccess) {
+            release();
+            return hip_fail(e, "autotune restore");
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    release();
+    h->itc = itc0;
+    if (best_ms) *best_ms = best_t;
     return LBM_OK;
 }
 

@@ -13,6 +13,19 @@
 
 namespace lbm {
 
+// Block shape and tile order of the step kernel.
+struct LaunchShape {
+    int kind = 0;          // 0: one node per thread; 1: z-marching (3D, FP32)
+    int bx = 64, by = 4;
+    int zchunk = 32;       // z planes per workgroup (kind 1)
+    int remap = 1;
+    int occ = 0;           // 3D FP32 kind 0: target waves per SIMD (0, 5 or 6)
+};
+
+// Whether the z-marching kernel has an instantiation for (bx, by) and its
+// wall stencils fit the staged tiles for this edge length.
+bool zmarch_supported(int n, int bx, int by);
+
 // Device-side state of one cavity.  Populations live in two ping-pong
 // buffers of (Qf + Qg) SoA planes each; the FP64 macroscopic fields are
 // written on the last step of every lbm_step call.
@@ -43,6 +56,7 @@ struct Ctx {
     int red_blocks = 0;
     int64_t itc = 0;
     bool initialized = false;
+    LaunchShape shape{};
     StepArgs<float> a32{};
     StepArgs<double> a64{};
 };
@@ -52,10 +66,12 @@ struct Ctx {
 void fill_args(Ctx &c);
 
 // Launchers (lbm_step_f32.hip / lbm_step_f64.hip).
-hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a, hipStream_t s);
+hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a,
+                           const LaunchShape &ls, hipStream_t s);
 hipError_t launch_prologue_f32(int dim, const StepArgs<float> &a,
                                const StepArgs<double> &ad, hipStream_t s);
-hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a, hipStream_t s);
+hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a,
+                           const LaunchShape &ls, hipStream_t s);
 hipError_t launch_prologue_f64(int dim, const StepArgs<double> &a,
                                const StepArgs<double> &ad, hipStream_t s);
 

@@ -66,6 +66,8 @@ template <typename R> struct StepArgs {
     R *__restrict__ force;  // FP64 path: per-node body force F(k), in place
     long plane;
     int n;
+    int gx, gy, ntiles;    // tile grid of the launch (x, y tiles; all tiles)
+    int remap;             // 1: XCD-aware tile order
     const R *__restrict__ tref;  // shifted path: Tref at each heated-axis index
     R dt, half_dt, gbeta;
     R tcorr;               // shifted path: -dt * dTref/dx (streaming of Tref)
@@ -124,6 +126,31 @@ template <int DIM> constexpr bool flow_row_active(int r) {
     else return r != 0 && r != 3 && r != 5;
 }
 
+// Opposite pairs (a, opp a) with a < opp a, in increasing a.  Every MRT row
+// is either even (same coefficient for a and opp a) or odd (opposite
+// coefficients) in the lattice velocity, so the FP32 path transforms pair
+// sums s = f_a + f_opp and differences d = f_a - f_opp.
+template <int DIM> constexpr int npairs() { return (Flow<DIM>::Q - 1) / 2; }
+template <int DIM> constexpr int pair_first(int p) {
+    int cnt = 0;
+    for (int a = 1; a < Flow<DIM>::Q; a++)
+        if (a < opposite<DIM>(a)) {
+            if (cnt == p) return a;
+            cnt++;
+        }
+    return -1;
+}
+template <int DIM> constexpr bool row_even(int r) {
+    for (int a = 1; a < Flow<DIM>::Q; a++)
+        if (Flow<DIM>::row(r, a) != Flow<DIM>::row(r, opposite<DIM>(a))) return false;
+    return true;
+}
+// Rows relaxed towards equilibrium (s_r != 0).
+template <int DIM> constexpr bool row_relaxed(int r) {
+    if constexpr (DIM == 2) return r != 0 && r != 3 && r != 5;
+    else return r != 0 && r != 3 && r != 5 && r != 7;
+}
+
 // m = M f, relax towards m_eq, add (I - S/2) M F_Guo dt, f_post = M^-1 m.
 // The body force is along +y (2D) or +z (3D) with magnitude Fb.
 template <int DIM, typename R, bool SHIFT>
@@ -132,19 +159,49 @@ __device__ __forceinline__ void collide_flow(const R *f, R base, R rho, R u, R v
                                              R *fp) {
     using FL = Flow<DIM>;
     constexpr int Q = FL::Q;
+    constexpr int NP = npairs<DIM>();
+    constexpr bool F64 = std::is_same<R, double>::value;
     R m[Q], me[Q], src[Q];
-    static_for<0, Q>([&](auto Rr) {
-        constexpr int r = decltype(Rr)::value;
-        R acc = R(0);
-        static_for<0, Q>([&](auto Aa) {
-            constexpr int al = decltype(Aa)::value;
-            constexpr double co = FL::row(r, al);
-            if constexpr (co == 1.0) acc += f[al];
-            else if constexpr (co == -1.0) acc -= f[al];
-            else if constexpr (co != 0.0) acc += R(co) * f[al];
+    R ps[NP], pd[NP];
+    if constexpr (F64) {
+        static_for<0, Q>([&](auto Rr) {
+            constexpr int r = decltype(Rr)::value;
+            R acc = R(0);
+            static_for<0, Q>([&](auto Aa) {
+                constexpr int al = decltype(Aa)::value;
+                constexpr double co = FL::row(r, al);
+                if constexpr (co == 1.0) acc += f[al];
+                else if constexpr (co == -1.0) acc -= f[al];
+                else if constexpr (co != 0.0) acc += R(co) * f[al];
+            });
+            m[r] = acc;
         });
-        m[r] = acc;
-    });
+    } else {
+        static_for<0, NP>([&](auto P) {
+            constexpr int p = decltype(P)::value;
+            constexpr int al = pair_first<DIM>(p), op = opposite<DIM>(al);
+            ps[p] = f[al] + f[op];
+            pd[p] = f[al] - f[op];
+        });
+        static_for<0, Q>([&](auto Rr) {
+            constexpr int r = decltype(Rr)::value;
+            if constexpr (row_relaxed<DIM>(r)) {
+                constexpr bool ev = row_even<DIM>(r);
+                constexpr double c0 = FL::row(r, 0);
+                R acc = R(0);
+                if constexpr (ev && c0 != 0.0) acc = R(c0) * f[0];
+                static_for<0, NP>([&](auto P) {
+                    constexpr int p = decltype(P)::value;
+                    constexpr double co = FL::row(r, pair_first<DIM>(p));
+                    const R x = ev ? ps[p] : pd[p];
+                    if constexpr (co == 1.0) acc += x;
+                    else if constexpr (co == -1.0) acc -= x;
+                    else if constexpr (co != 0.0) acc += R(co) * x;
+                });
+                m[r] = acc;
+            }
+        });
+    }
     flow_meq<DIM, R>(base, rho, u, v, w, me);
     R s[Q];
     if constexpr (DIM == 2) {
@@ -207,22 +264,38 @@ __device__ __forceinline__ void collide_flow(const R *f, R base, R rho, R u, R v
         // instead of being scaled every step by the rounded 1/|row|^2.
         static_for<0, Q>([&](auto Rr) {
             constexpr int r = decltype(Rr)::value;
-            if constexpr (flow_row_active<DIM>(r))
-                m[r] = (src[r] * a.dt - s[r] * (m[r] - me[r])) * R(1.0 / flow_norm<DIM>(r));
+            if constexpr (flow_row_active<DIM>(r)) {
+                if constexpr (row_relaxed<DIM>(r))
+                    m[r] = (src[r] * a.dt - s[r] * (m[r] - me[r])) * R(1.0 / flow_norm<DIM>(r));
+                else
+                    m[r] = src[r] * a.dt * R(1.0 / flow_norm<DIM>(r));
+            }
         });
-        static_for<0, Q>([&](auto Aa) {
-            constexpr int al = decltype(Aa)::value;
-            R acc = R(0);
+        // inverse transform on pairs: f_a += E + O, f_opp += E - O with E the
+        // even rows' and O the odd rows' contribution
+        R e0 = R(0);
+        static_for<0, Q>([&](auto Rr) {
+            constexpr int r = decltype(Rr)::value;
+            constexpr double c0 = FL::row(r, 0);
+            if constexpr (flow_row_active<DIM>(r) && c0 != 0.0) e0 += R(c0) * m[r];
+        });
+        fp[0] = f[0] + e0;
+        static_for<0, NP>([&](auto P) {
+            constexpr int p = decltype(P)::value;
+            constexpr int al = pair_first<DIM>(p), op = opposite<DIM>(al);
+            R ev = R(0), od = R(0);
             static_for<0, Q>([&](auto Rr) {
                 constexpr int r = decltype(Rr)::value;
                 constexpr double co = FL::row(r, al);
-                if constexpr (flow_row_active<DIM>(r)) {
+                if constexpr (flow_row_active<DIM>(r) && co != 0.0) {
+                    R &acc = row_even<DIM>(r) ? ev : od;
                     if constexpr (co == 1.0) acc += m[r];
                     else if constexpr (co == -1.0) acc -= m[r];
-                    else if constexpr (co != 0.0) acc += R(co) * m[r];
+                    else acc += R(co) * m[r];
                 }
             });
-            fp[al] = f[al] + acc;
+            fp[al] = f[al] + (ev + od);
+            fp[op] = f[op] + (ev - od);
         });
     }
 }
@@ -342,69 +415,124 @@ __device__ __forceinline__ R lerp3(const R *w, R f0, R f1, R f2) {
     else return f1 + w[0] * (f0 - f1) + w[2] * (f2 - f1);
 }
 
+// Element idx of a population plane: a uniform 64-bit base plus a 32-bit
+// byte offset, so the load is a scalar-base + vector-offset access and the
+// stencil offsets common to all populations are formed once per node.  The
+// caller bounds a plane to fewer than 2^32 bytes.
+template <typename R>
+__device__ __forceinline__ R ldp(const R *__restrict__ base, uint32_t idx) {
+    return *reinterpret_cast<const R *>(reinterpret_cast<const char *>(base) +
+                                        idx * (uint32_t)sizeof(R));
+}
+template <typename R>
+__device__ __forceinline__ void stp(R *__restrict__ base, uint32_t idx, R v) {
+    *reinterpret_cast<R *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(R)) = v;
+}
+
 // One population: a 1D interpolation when one velocity component is
 // non-zero, a two-stage tensor interpolation (first along the higher axis,
 // then along the lower one) when two are.
 template <int CX, int CY, int CZ, typename R>
-__device__ __forceinline__ R interp_pop(const R *__restrict__ F, long node, R fc,
-                                        const AxisStencil<R> *st, const long *stride) {
+__device__ __forceinline__ R interp_pop(const R *__restrict__ F, uint32_t node, R fc,
+                                        const AxisStencil<R> *st, const uint32_t *stride) {
     constexpr int c[3] = {CX, CY, CZ};
     constexpr int nz = (CX != 0) + (CY != 0) + (CZ != 0);
     if constexpr (nz == 1) {
         constexpr int ax = CX ? 0 : (CY ? 1 : 2);
         const AxisStencil<R> &s = st[ax];
         const R *w = c[ax] > 0 ? s.wp : s.wm;
-        return lerp3(w, F[node + s.o0 * stride[ax]], fc, F[node + s.o2 * stride[ax]]);
+        return lerp3(w, ldp(F, node + (uint32_t)s.o0 * stride[ax]), fc,
+                     ldp(F, node + (uint32_t)s.o2 * stride[ax]));
     } else {
         constexpr int q = CX ? 0 : 1;          // second (outer) stage axis
         constexpr int p = CZ ? 2 : 1;          // first (inner) stage axis
         const AxisStencil<R> &sp = st[p], &sq = st[q];
         const R *wpp = c[p] > 0 ? sp.wp : sp.wm;
         const R *wqq = c[q] > 0 ? sq.wp : sq.wm;
-        const long dp0 = sp.o0 * stride[p], dp2 = sp.o2 * stride[p];
-        const long n0 = node + sq.o0 * stride[q], n2 = node + sq.o2 * stride[q];
-        R c0 = lerp3(wpp, F[n0 + dp0], F[n0], F[n0 + dp2]);
-        R c1 = lerp3(wpp, F[node + dp0], fc, F[node + dp2]);
-        R c2 = lerp3(wpp, F[n2 + dp0], F[n2], F[n2 + dp2]);
+        const uint32_t dp0 = (uint32_t)sp.o0 * stride[p], dp2 = (uint32_t)sp.o2 * stride[p];
+        const uint32_t n0 = node + (uint32_t)sq.o0 * stride[q];
+        const uint32_t n2 = node + (uint32_t)sq.o2 * stride[q];
+        R c0 = lerp3(wpp, ldp(F, n0 + dp0), ldp(F, n0), ldp(F, n0 + dp2));
+        R c1 = lerp3(wpp, ldp(F, node + dp0), fc, ldp(F, node + dp2));
+        R c2 = lerp3(wpp, ldp(F, n2 + dp0), ldp(F, n2), ldp(F, n2 + dp2));
         return lerp3(wqq, c0, c1, c2);
     }
 }
 
 // ---------------------------------------------------------------- step
-template <int DIM, typename R, bool SHIFT, bool MACRO>
-__global__ __launch_bounds__(256) void k_step(StepArgs<R> a) {
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  With remap on, hardware workgroup P runs logical tile
+// (P % 8) * (G/8) + P/8, so each XCD sweeps one contiguous range of tiles and
+// the neighbour tiles whose rows it re-reads sit in its own L2.  The tail
+// G % 8 tiles keep their identity.
+__device__ __forceinline__ int logical_tile(int P, int G, int remap) {
+    if (remap != 1) return P;
+    const int g8 = G >> 3;
+    if (P >= (g8 << 3)) return P;
+    return (P & 7) * g8 + (P >> 3);
+}
+
+// Tile coordinates (tx, ty, tz) of hardware workgroup P.  remap 0 and 1 use
+// the x-fastest order of logical_tile.  remap 2 (needs gy % 8 == 0) gives XCD
+// P % 8 the band of gy/8 tile rows [x*gy/8, (x+1)*gy/8) and sweeps it
+// x fastest, then y, then z: x and y neighbour tiles share the XCD's L2 and
+// the tile above in z comes gx*gy/8 workgroups later on the same XCD.
+__device__ __forceinline__ void tile_coords(int P, int G, int gx, int gy, int remap, int &tx,
+                                            int &ty, int &tz) {
+    if (remap == 2) {
+        const int band = gy >> 3, q = P >> 3;
+        tx = q % gx;
+        const int r = q / gx;
+        ty = (P & 7) * band + r % band;
+        tz = r / band;
+        return;
+    }
+    const int L = logical_tile(P, G, remap);
+    tx = L % gx;
+    const int r = L / gx;
+    ty = r % gy;
+    tz = r / gy;
+}
+
+// WPE > 0 asks the compiler for at least WPE resident waves per SIMD (it
+// then caps the registers per lane), trading load hoisting for latency
+// hiding; the autotuner picks.
+template <int DIM, typename R, bool SHIFT, bool MACRO, int WPE = 0>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_step(StepArgs<R> a) {
     using FL = Flow<DIM>;
     using HT = Heat<DIM>;
     constexpr int QF = FL::Q, QG = HT::Q;
     const int n = a.n;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = DIM == 3 ? (int)blockIdx.z : 0;
+    int tx, ty, tz;
+    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tz);
+    const int i = tx * blockDim.x + threadIdx.x;
+    const int j = ty * blockDim.y + threadIdx.y;
+    const int k = DIM == 3 ? tz : 0;
     if (i >= n || j >= n) return;
-    const long sy = n, sz = (long)n * n;
-    const long node = i + sy * j + sz * k;
+    const uint32_t sy = n, sz = (uint32_t)n * n;
+    const uint32_t node = i + sy * j + sz * k;
     const int idx[3] = {i, j, k};
-    const long stride[3] = {1, sy, sz};
+    const uint32_t stride[3] = {1, sy, sz};
     AxisStencil<R> st[3];
     st[0] = load_stencil(a, i);
     st[1] = load_stencil(a, j);
     if constexpr (DIM == 3) st[2] = load_stencil(a, k);
 
-    // own-node post-collision values (the stencils' centre points)
     R pcf[QF], pcg[QG], f[QF], g[QG];
     static_for<0, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        pcf[al] = a.pin[al * a.plane + node];
+        pcf[al] = ldp(a.pin + al * a.plane, node);
     });
     static_for<0, QG>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        pcg[al] = a.pin[(QF + al) * a.plane + node];
+        pcg[al] = ldp(a.pin + (QF + al) * a.plane, node);
     });
-
+    R sp_f = R(0), sp_g = R(0);
+    static_for<0, QF>([&](auto A) { sp_f += pcf[decltype(A)::value]; });
+    static_for<0, QG>([&](auto A) { sp_g += pcg[decltype(A)::value]; });
     constexpr int HAX = heated_axis<DIM>();
     const R tloc = SHIFT ? a.tref[idx[HAX]] : R(0);
-
-    // 1. interpolation-supplemented streaming
     f[0] = pcf[0];
     g[0] = pcg[0];
     static_for<1, QF>([&](auto A) {
@@ -421,9 +549,6 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<R> a) {
             g[al] += R(wc) * a.tcorr;
         }
     });
-
-    // 2. walls: populations entering the fluid take the value of their
-    // opposite at the same node; heated walls use anti-bounce-back.
     static_for<0, DIM>([&](auto AX) {
         constexpr int ax = decltype(AX)::value;
         const int side = idx[ax] == 0 ? 1 : (idx[ax] == n - 1 ? -1 : 0);
@@ -460,39 +585,65 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<R> a) {
     // zeroth moments of the node's own P(k).
     R F_prev;
     if constexpr (std::is_same<R, double>::value) {
-        F_prev = a.force[node];
+        F_prev = ldp(a.force, node);
     } else {
-        R sp_f = R(0), sp_g = R(0);
-        static_for<0, QF>([&](auto A) { sp_f += pcf[decltype(A)::value]; });
-        static_for<0, QG>([&](auto A) { sp_g += pcg[decltype(A)::value]; });
         const R rho_prev = SHIFT ? R(1) + sp_f : sp_f;
         const R T_prev = SHIFT ? tloc + sp_g : sp_g;
         F_prev = a.gbeta * T_prev * rho_prev;
     }
 
     R srho = R(0), jx = R(0), jy = R(0), jz = R(0), sT = R(0);
-    static_for<0, QF>([&](auto A) { srho += f[decltype(A)::value]; });
-    static_for<1, QF>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        if constexpr (FL::c[al][0] > 0) jx += f[al];
-        if constexpr (FL::c[al][0] < 0) jx -= f[al];
-        if constexpr (FL::c[al][1] > 0) jy += f[al];
-        if constexpr (FL::c[al][1] < 0) jy -= f[al];
-        if constexpr (FL::c[al][2] > 0) jz += f[al];
-        if constexpr (FL::c[al][2] < 0) jz -= f[al];
-    });
+    if constexpr (std::is_same<R, double>::value) {
+        static_for<0, QF>([&](auto A) { srho += f[decltype(A)::value]; });
+        static_for<1, QF>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            if constexpr (FL::c[al][0] > 0) jx += f[al];
+            if constexpr (FL::c[al][0] < 0) jx -= f[al];
+            if constexpr (FL::c[al][1] > 0) jy += f[al];
+            if constexpr (FL::c[al][1] < 0) jy -= f[al];
+            if constexpr (FL::c[al][2] > 0) jz += f[al];
+            if constexpr (FL::c[al][2] < 0) jz -= f[al];
+        });
+    } else {
+        // on opposite pairs, shared with the collision's moment transform
+        srho = f[0];
+        static_for<0, npairs<DIM>()>([&](auto P) {
+            constexpr int p = decltype(P)::value;
+            constexpr int al = pair_first<DIM>(p), op = opposite<DIM>(al);
+            const R sm = f[al] + f[op], df = f[al] - f[op];
+            srho += sm;
+            if constexpr (FL::c[al][0] > 0) jx += df;
+            if constexpr (FL::c[al][0] < 0) jx -= df;
+            if constexpr (FL::c[al][1] > 0) jy += df;
+            if constexpr (FL::c[al][1] < 0) jy -= df;
+            if constexpr (FL::c[al][2] > 0) jz += df;
+            if constexpr (FL::c[al][2] < 0) jz -= df;
+        });
+    }
     static_for<0, QG>([&](auto A) { sT += g[decltype(A)::value]; });
     const R rho = SHIFT ? R(1) + srho : srho;
     const R T = SHIFT ? tloc + sT : sT;
     R ux, uy, uz;
-    if constexpr (DIM == 2) {
-        ux = jx / rho;
-        uy = (jy + a.half_dt * F_prev) / rho;
-        uz = R(0);
+    if constexpr (std::is_same<R, double>::value) {
+        if constexpr (DIM == 2) {
+            ux = jx / rho;
+            uy = (jy + a.half_dt * F_prev) / rho;
+            uz = R(0);
+        } else {
+            ux = jx / rho;
+            uy = jy / rho;
+            uz = (jz + a.half_dt * F_prev) / rho;
+        }
     } else {
-        ux = jx / rho;
-        uy = jy / rho;
-        uz = (jz + a.half_dt * F_prev) / rho;
+        const R irho = R(1) / rho;
+        ux = jx * irho;
+        if constexpr (DIM == 2) {
+            uy = (jy + a.half_dt * F_prev) * irho;
+            uz = R(0);
+        } else {
+            uy = jy * irho;
+            uz = (jz + a.half_dt * F_prev) * irho;
+        }
     }
     if constexpr (MACRO) {
         if constexpr (SHIFT) {
@@ -513,17 +664,17 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<R> a) {
 
     // 4. collision with the force of this step
     const R Fb = a.gbeta * T * rho;
-    if constexpr (std::is_same<R, double>::value) a.force[node] = Fb;
+    if constexpr (std::is_same<R, double>::value) stp(a.force, node, Fb);
     R fp[QF], gp[QG];
     collide_flow<DIM, R, SHIFT>(f, SHIFT ? srho : rho, rho, ux, uy, uz, Fb, a, fp);
     collide_heat<DIM, R>(g, SHIFT ? sT : T, T, ux, uy, uz, a, gp);
     static_for<0, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        a.pout[al * a.plane + node] = fp[al];
+        stp(a.pout + al * a.plane, node, fp[al]);
     });
     static_for<0, QG>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        a.pout[(QF + al) * a.plane + node] = gp[al];
+        stp(a.pout + (QF + al) * a.plane, node, gp[al]);
     });
 }
 

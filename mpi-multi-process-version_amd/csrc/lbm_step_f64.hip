@@ -6,9 +6,15 @@
 
 namespace lbm {
 
-hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a, hipStream_t s) {
-    dim3 block(64, 4, 1);
-    dim3 grid((a.n + 63) / 64, (a.n + 3) / 4, dim == 3 ? a.n : 1);
+hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a0, const LaunchShape &ls,
+                           hipStream_t s) {
+    StepArgs<double> a = a0;
+    dim3 block(ls.bx, ls.by, 1);
+    a.gx = (a.n + ls.bx - 1) / ls.bx;
+    a.gy = (a.n + ls.by - 1) / ls.by;
+    a.ntiles = a.gx * a.gy * (dim == 3 ? a.n : 1);
+    a.remap = (ls.remap == 2 && a.gy % 8) ? 0 : ls.remap;
+    dim3 grid(a.ntiles);
     if (dim == 2) {
         if (macro) hipLaunchKernelGGL((k_step<2, double, false, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((k_step<2, double, false, false>), grid, block, 0, s, a);

@@ -35,7 +35,8 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_get_grid", "lbm_get_info",
            "lbm_init_rest", "lbm_init_fields", "lbm_step", "lbm_check",
            "lbm_nusselt", "lbm_get_fields", "lbm_run", "lbm_get_itc",
            "lbm_time_steps", "lbm_synchronize", "lbm_state_size",
-           "lbm_save_state", "lbm_load_state", "lbm_bytes_per_update",
+           "lbm_save_state", "lbm_load_state", "lbm_set_launch",
+           "lbm_get_launch", "lbm_autotune", "lbm_bytes_per_update",
            "lbm_status_string", "lbm_last_error"]
 
 
@@ -43,6 +44,11 @@ class LbmError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{msg} (status {code})")
         self.code = code
+
+
+class Launch(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in
+                ("kind", "bx", "by", "zchunk", "remap", "occupancy")]
 
 
 class Params(ctypes.Structure):
@@ -91,6 +97,9 @@ def load():
             "lbm_save_state": [H, ctypes.c_void_p, I64],
             "lbm_load_state": [H, ctypes.c_void_p, I64],
             "lbm_bytes_per_update": [H, DP],
+            "lbm_set_launch": [H, ctypes.POINTER(Launch)],
+            "lbm_get_launch": [H, ctypes.POINTER(Launch)],
+            "lbm_autotune": [H, I32, DP],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -221,6 +230,22 @@ class Solver:
         b = ctypes.c_double()
         _check(self._L.lbm_bytes_per_update(self.h, ctypes.byref(b)))
         return b.value
+
+    def set_launch(self, bx, by, remap=1, kind=0, zchunk=32, occupancy=0):
+        cfg = Launch(kind=kind, bx=bx, by=by, zchunk=zchunk, remap=remap,
+                     occupancy=occupancy)
+        _check(self._L.lbm_set_launch(self.h, ctypes.byref(cfg)))
+
+    def get_launch(self):
+        """dict of the launch configuration that runs."""
+        cfg = Launch()
+        _check(self._L.lbm_get_launch(self.h, ctypes.byref(cfg)))
+        return {k: getattr(cfg, k) for k, _ in Launch._fields_}
+
+    def autotune(self, steps_per_trial=10):
+        ms = ctypes.c_double()
+        _check(self._L.lbm_autotune(self.h, int(steps_per_trial), ctypes.byref(ms)))
+        return ms.value
 
     def save_state(self):
         size = ctypes.c_int64()
