@@ -70,6 +70,8 @@ def main():
                     help="oracle steps for the CPU baseline (0: automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch", default="",
+                    help="fixed launch shape bx,by,remap,occupancy (implies no autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (5 times), the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -93,10 +95,13 @@ def main():
     prec = lbmhip.PREC_FP64 if args.precision == "fp64" else lbmhip.PREC_FP32_SHIFTED
     s = lbmhip.Solver(cfg["dim"], cfg["n"], cfg["Ra"], precision=prec, device=0)
     s.init_rest()
+    if args.launch:
+        bx, by, remap, occ = (int(v) for v in args.launch.split(","))
+        s.set_launch(bx, by, remap=remap, occupancy=occ)
+    elif not args.no_autotune:
+        s.autotune(10)   # restores the state it started from
     if args.warmup:
         s.step(args.warmup)
-    if not args.no_autotune:
-        s.autotune(10)   # restores the state it started from
     launch = s.get_launch()
     s.synchronize()
     t0 = time.perf_counter()
@@ -115,7 +120,7 @@ def main():
 
     traffic = traffic_from_profiles(args.workload)
     out = {
-        "metric": "MLUPS (million lattice-node updates per second), fused MRT-LBM step",
+        "metric": "MLUPS + % HBM peak (roofline.frac), fused MRT-LBM step, 1 GPU",
         "value": round(mlups, 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -140,7 +145,7 @@ def main():
         "check": {"errU": eu, "errT": et, "nu_hot": nu[0], "nu_cold": nu[1]},
     }
     if not args.no_cpu_baseline:
-        steps = args.cpu_steps or (2 if cfg["dim"] == 3 else 40)
+        steps = args.cpu_steps or (10 if cfg["dim"] == 3 else 300)
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_threads, steps)
     else:
         out["cpu_baseline"] = None

@@ -47,7 +47,9 @@ extern "C" {
 #define LBM_GRID_ERF   0  /* x_i = (erf(A(i/(N+1)-1/2))/erf(A/2)+1)/2        */
 #define LBM_GRID_TANH  1  /* same with tanh in place of erf (extension)      */
 
-#define LBM_PREC_FP32_SHIFTED  0  /* FP32 populations stored as f - w*rho0    */
+#define LBM_PREC_FP32_SHIFTED  0  /* FP32 populations stored as deviations from
+                                     the rest equilibrium (flow) and from the
+                                     conduction profile (heat)                */
 #define LBM_PREC_FP64          1  /* FP64 populations, the reference's form   */
 
 /* Inputs of the reference's `commondata` module, made run-time
@@ -166,25 +168,20 @@ int32_t lbm_save_state(lbm_handle h, void *buf, int64_t bytes);
 int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes);
 
 /* Launch configuration of the step kernel (no reference counterpart: the
- * reference's loops have no launch shape).
- *  kind 0: one lattice node per thread, bx x by threads per workgroup.
- *  kind 1 (3D, FP32 only): each workgroup owns a bx x by tile of columns and
- *    marches zchunk planes along z with the plane tiles staged in LDS (shapes
- *    64x4, 64x8, 128x2, 128x4, 32x8, 96x6; other shapes or unsupported edge
- *    lengths run kind 0).
+ * reference's loops have no launch shape).  One lattice node per thread,
+ * bx x by threads per workgroup (bx a multiple of 32, bx*by a multiple of 64
+ * and at most 1024).
  *  remap: tile order.  Workgroups are dealt round-robin to the 8 XCDs, each
  *    with its own L2.  0 = x fastest, then y, then z; 1 = each XCD sweeps one
  *    contiguous range of that order; 2 = each XCD owns a band of tile rows in
  *    y and sweeps it plane by plane (needs the number of y tiles divisible by
  *    8, else 0 is used).
- *  occupancy (3D FP32 kind 0): 0, or 5 or 6 resident waves per SIMD, which
- *    caps the registers per lane.
- * bx must be a multiple of 32, bx*by a multiple of 64 and at most 1024. */
+ *  occupancy (3D, FP32 storage): 0, or 5 or 6 resident waves per SIMD, which
+ *    caps the registers per lane. */
 typedef struct lbm_launch {
-    int32_t kind, bx, by, zchunk, remap, occupancy;
+    int32_t bx, by, remap, occupancy;
 } lbm_launch;
 int32_t lbm_set_launch(lbm_handle h, const lbm_launch *launch);
-/* The configuration that runs (kind 0 if kind 1 does not apply). */
 int32_t lbm_get_launch(lbm_handle h, lbm_launch *launch);
 
 /* Block-shape autotuner: times steps_per_trial steps for each candidate

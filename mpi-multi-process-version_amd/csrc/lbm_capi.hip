@@ -111,7 +111,8 @@ double cv_width(const Ctx &c, int j) {
     return 0.5 * (c.x[j + 1] - c.x[j - 1]);
 }
 
-template <typename R> void set_consts(const Ctx &c, lbm::StepArgs<R> &a, bool shifted) {
+template <typename A> void set_consts(const Ctx &c, A &a, bool shifted) {
+    using R = typename A::real;
     const double Snu = c.info[LBM_INFO_SNU], Sq = c.info[LBM_INFO_SQ];
     const double dt = c.info[LBM_INFO_DT];
     a.n = c.n;
@@ -122,12 +123,12 @@ template <typename R> void set_consts(const Ctx &c, lbm::StepArgs<R> &a, bool sh
     if (shifted) {
         // conduction profile Tref(x) = Thot + (Tcold - Thot) x/H, stored in
         // FP32; the wall values are taken relative to the stored numbers
-        const double slope = (c.p.Tcold - c.p.Thot) / c.info[LBM_INFO_H];
-        a.tcorr = R(-dt * slope);
-        a.tw_hot = R(c.p.Thot - (double)(float)c.tref[0]);
-        a.tw_cold = R(c.p.Tcold - (double)(float)c.tref[c.n - 1]);
+        const bool f32 = std::is_same<R, float>::value;
+        const double t0 = f32 ? (double)(float)c.tref[0] : c.tref[0];
+        const double t1 = f32 ? (double)(float)c.tref[c.n - 1] : c.tref[c.n - 1];
+        a.tw_hot = R(c.p.Thot - t0);
+        a.tw_cold = R(c.p.Tcold - t1);
     } else {
-        a.tcorr = R(0);
         a.tw_hot = R(c.p.Thot);
         a.tw_cold = R(c.p.Tcold);
     }
@@ -159,7 +160,7 @@ template <typename R> void set_consts(const Ctx &c, lbm::StepArgs<R> &a, bool sh
 
 void free_ctx(Ctx *c) {
     void *ptrs[] = {c->pbuf[0], c->pbuf[1], c->rho, c->u, c->v, c->w, c->T, c->up, c->vp,
-                    c->wp, c->Tp, c->force, c->d_off, c->d_wt, c->d_x, c->d_cvw, c->d_tref,
+                    c->wp, c->Tp, c->force, c->d_off, c->d_wt, c->d_x, c->d_cvw, c->d_tref, c->d_tcorr,
                     c->d_part, c->d_red};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -296,6 +297,7 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
     ALLOC(c->pbuf[1], pbytes);
     const size_t fbytes = sizeof(double) * (size_t)c->nn;
     ALLOC(c->d_tref, sizeof(float) * (size_t)c->n);
+    ALLOC(c->d_tcorr, sizeof(float) * 2 * (size_t)c->n);
     ALLOC(c->rho, fbytes);
     ALLOC(c->u, fbytes);
     ALLOC(c->v, fbytes);
@@ -309,7 +311,7 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
     }
     if (c->fp64) ALLOC(c->force, fbytes);
     ALLOC(c->d_off, sizeof(int2) * (size_t)c->n);
-    ALLOC(c->d_wt, esz * 6 * (size_t)c->n);
+    ALLOC(c->d_wt, (c->fp64 ? sizeof(double) : sizeof(float)) * 6 * (size_t)c->n);
     ALLOC(c->d_x, sizeof(double) * (size_t)c->n);
     ALLOC(c->d_cvw, sizeof(double) * (size_t)c->n);
     long want = (c->nn + lbm::kRedThreads - 1) / lbm::kRedThreads;
@@ -331,7 +333,23 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
     for (int i = 0; i < c->n; i++)
         c->tref[i] = p.Thot + (p.Tcold - p.Thot) * c->x[i] / c->info[LBM_INFO_H];
     std::vector<float> treff(c->tref.begin(), c->tref.end());
-    if ((e = hipMemcpy(c->d_tref, treff.data(), sizeof(float) * c->n, hipMemcpyHostToDevice)))
+    // Streaming of the stored profile: with the FP32 weights W0, W2 used by
+    // the kernel's difference-form interpolation, the shift w*Tref streams
+    // to w*(Tref_i + W0 (Tref_0 - Tref_i) + W2 (Tref_2 - Tref_i)); the
+    // bracket minus Tref_i is formed here in FP64 from the stored values.
+    std::vector<float> tcorr(2 * (size_t)c->n);
+    for (int i = 0; i < c->n; i++) {
+        const int *t = &c->st[3 * (size_t)i];
+        const double ti = treff[i], t0 = treff[t[0]], t2 = treff[t[2]];
+        for (int s = 0; s < 2; s++) {
+            const double w0 = (double)(float)wt[6 * (size_t)i + 3 * s];
+            const double w2 = (double)(float)wt[6 * (size_t)i + 3 * s + 2];
+            tcorr[2 * (size_t)i + s] = (float)(w0 * (t0 - ti) + w2 * (t2 - ti));
+        }
+    }
+    if ((e = hipMemcpy(c->d_tref, treff.data(), sizeof(float) * c->n, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(c->d_tcorr, tcorr.data(), sizeof(float) * tcorr.size(),
+                       hipMemcpyHostToDevice)))
         return cleanup(hip_fail(e, "tref upload"));
     std::vector<double> cvw(c->n);
     for (int i = 0; i < c->n; i++) cvw[i] = cv_width(*c, i) / c->info[LBM_INFO_H];
@@ -347,16 +365,19 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
     }
     if (e != hipSuccess) return cleanup(hip_fail(e, "weight upload"));
 
-    set_consts<double>(*c, c->a64, false);
+    set_consts(*c, c->a64, false);
     c->a64.off = c->d_off;
     c->a64.wt = (const double *)(c->fp64 ? c->d_wt : nullptr);
     c->a64.force = c->force;
-    set_consts<float>(*c, c->a32, true);
+    set_consts(*c, c->a32, true);
     c->a32.off = c->d_off;
     c->a32.wt = (const float *)(c->fp64 ? nullptr : c->d_wt);
     c->a32.force = nullptr;
     c->a32.tref = c->d_tref;
+    c->a32.tcorr = c->d_tcorr;
+    c->a64.tcorr = nullptr;
     c->a64.tref = nullptr;
+
     *out = c;
     g_err.clear();
     return LBM_OK;
@@ -611,14 +632,12 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes) {
 
 int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
     if (!h || !l) return fail(LBM_ERR_ARG, "null argument");
-    if ((l->kind != 0 && l->kind != 1) || l->bx < 32 || l->bx % 32 || l->by < 1 ||
-        l->bx * l->by > 1024 || (l->bx * l->by) % 64 || l->remap < 0 || l->remap > 2 ||
-        l->zchunk < 1 || (l->occupancy != 0 && l->occupancy != 5 && l->occupancy != 6))
+    if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
+        l->remap < 0 || l->remap > 2 ||
+        (l->occupancy != 0 && l->occupancy != 5 && l->occupancy != 6))
         return fail(LBM_ERR_ARG, "invalid launch configuration");
-    h->shape.kind = l->kind;
     h->shape.bx = l->bx;
     h->shape.by = l->by;
-    h->shape.zchunk = l->zchunk;
     h->shape.remap = l->remap;
     h->shape.occ = l->occupancy;
     return LBM_OK;
@@ -626,14 +645,10 @@ int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
 
 int32_t lbm_get_launch(lbm_handle h, lbm_launch *l) {
     if (!h || !l) return fail(LBM_ERR_ARG, "null argument");
-    const bool z = h->shape.kind == 1 && h->dim == 3 && !h->fp64 &&
-                   lbm::zmarch_supported(h->n, h->shape.bx, h->shape.by);
-    l->kind = z ? 1 : 0;
     l->bx = h->shape.bx;
     l->by = h->shape.by;
-    l->zchunk = h->shape.zchunk;
     l->remap = h->shape.remap;
-    l->occupancy = (!z && h->dim == 3 && !h->fp64) ? h->shape.occ : 0;
+    l->occupancy = (h->dim == 3 && h->p.precision == LBM_PREC_FP32_SHIFTED) ? h->shape.occ : 0;
     return LBM_OK;
 }
 
@@ -662,25 +677,20 @@ int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms) {
     const int cur0 = h->cur;
     const int64_t itc0 = h->itc;
     const lbm::LaunchShape start = h->shape;
-    // candidates: (kind, bx, by, zchunk); each with both tile orders
-    static const int cand[][4] = {{0, 64, 1, 0},  {0, 64, 2, 0},  {0, 64, 4, 0},  {0, 128, 1, 0},
-                                  {0, 128, 2, 0}, {0, 128, 4, 0}, {0, 256, 1, 0}, {0, 256, 2, 0},
-                                  {0, 32, 8, 0},  {0, 64, 8, 0}};
+    // candidates (bx, by), each with every tile order and, for the 3D FP32
+    // kernel, each occupancy target
+    static const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {64, 8}, {128, 1},
+                                  {128, 2}, {128, 4}, {256, 1}, {256, 2}, {32, 8}};
     lbm::LaunchShape best = start;
     double best_t = 1e300;
+    const int nocc = (h->dim == 3 && h->p.precision == LBM_PREC_FP32_SHIFTED) ? 3 : 1;
     for (const auto &cd : cand) {
-        const bool z = cd[0] == 1;
-        if (z && (h->dim != 3 || h->fp64 || !lbm::zmarch_supported(h->n, cd[1], cd[2]))) continue;
-        if (!z && cd[1] > 2 * h->n && cd[1] > 64) continue;
-        const int nocc = (h->dim == 3 && !h->fp64 && !z) ? 3 : 1;
+        if (cd[0] > 2 * h->n && cd[0] > 64) continue;
         for (int combo = 0; combo < 3 * nocc; combo++) {
-            const int remap = combo % 3;
             static const int occs[3] = {0, 5, 6};
-            h->shape.kind = cd[0];
-            h->shape.bx = cd[1];
-            h->shape.by = cd[2];
-            h->shape.zchunk = z ? cd[3] : 32;
-            h->shape.remap = remap;
+            h->shape.bx = cd[0];
+            h->shape.by = cd[1];
+            h->shape.remap = combo % 3;
             h->shape.occ = occs[combo / 3];
             double ms = 0;
             // one untimed warm-up launch per candidate, then the timed trial

@@ -15,16 +15,10 @@ namespace lbm {
 
 // Block shape and tile order of the step kernel.
 struct LaunchShape {
-    int kind = 0;          // 0: one node per thread; 1: z-marching (3D, FP32)
     int bx = 64, by = 4;
-    int zchunk = 32;       // z planes per workgroup (kind 1)
     int remap = 1;
-    int occ = 0;           // 3D FP32 kind 0: target waves per SIMD (0, 5 or 6)
+    int occ = 0;           // 3D FP32: target waves per SIMD (0, 5 or 6)
 };
-
-// Whether the z-marching kernel has an instantiation for (bx, by) and its
-// wall stencils fit the staged tiles for this edge length.
-bool zmarch_supported(int n, int bx, int by);
 
 // Device-side state of one cavity.  Populations live in two ping-pong
 // buffers of (Qf + Qg) SoA planes each; the FP64 macroscopic fields are
@@ -33,7 +27,7 @@ struct Ctx {
     lbm_params p{};
     int dim = 2, n = 0, qf = 9, qg = 5;
     long nn = 0, plane = 0;
-    bool fp64 = false;
+    bool fp64 = false;          // FP64 storage and arithmetic
     double info[LBM_INFO_COUNT] = {};
     double dx0 = 0;
     std::vector<double> x;      // node coordinates (LB units), same on every axis
@@ -51,6 +45,7 @@ struct Ctx {
     double *d_x = nullptr;      // node coordinates
     double *d_cvw = nullptr;    // control-volume widths / H
     float *d_tref = nullptr;    // FP32 conduction profile (shifted storage)
+    float *d_tcorr = nullptr;   // its streaming correction, 2 per index
     double *d_part = nullptr;   // reduction partials
     double *d_red = nullptr;    // reduction results
     int red_blocks = 0;

@@ -53,9 +53,12 @@ __device__ __forceinline__ void static_for(F &&f) {
 // Per-launch constants.  Coefficients that depend only on the relaxation
 // rates are formed on the host in FP64, in the same operation order as the
 // oracle, and passed in.
-template <typename R> struct StepArgs {
-    const R *__restrict__ pin;
-    R *__restrict__ pout;
+// R is the arithmetic type and S the storage type of the populations.
+template <typename R, typename S = R> struct StepArgs {
+    using real = R;
+    using store = S;
+    const S *__restrict__ pin;
+    S *__restrict__ pout;
     const int2 *__restrict__ off;  // per grid index: offsets of stencil points 0 and 2
     const R *__restrict__ wt;      // per grid index: w(+1)[3], w(-1)[3]
     double *__restrict__ rho;
@@ -70,7 +73,10 @@ template <typename R> struct StepArgs {
     int remap;             // 1: XCD-aware tile order
     const R *__restrict__ tref;  // shifted path: Tref at each heated-axis index
     R dt, half_dt, gbeta;
-    R tcorr;               // shifted path: -dt * dTref/dx (streaming of Tref)
+    // shifted path, per heated-axis index: interpolation of the stored Tref
+    // minus Tref at the node, for c = +1 and c = -1 (exact for the stored
+    // FP32 weights and Tref values, formed in FP64 on the host)
+    const R *__restrict__ tcorr;
     R tw_hot, tw_cold;     // T_wall - Tref(wall node) (plain path: T_wall)
     R s_nu, s_q, sig_k, q_e;
     // Guo source coefficients (1 - s/2) times the lattice factor
@@ -153,17 +159,17 @@ template <int DIM> constexpr bool row_relaxed(int r) {
 
 // m = M f, relax towards m_eq, add (I - S/2) M F_Guo dt, f_post = M^-1 m.
 // The body force is along +y (2D) or +z (3D) with magnitude Fb.
-template <int DIM, typename R, bool SHIFT>
+// EXACT selects the operation order of the FP64 oracle (FP64 storage);
+// otherwise the increment form on opposite pairs is used.
+template <int DIM, typename R, bool EXACT, typename A>
 __device__ __forceinline__ void collide_flow(const R *f, R base, R rho, R u, R v,
-                                             R w, R Fb, const StepArgs<R> &a,
-                                             R *fp) {
+                                             R w, R Fb, const A &a, R *fp) {
     using FL = Flow<DIM>;
     constexpr int Q = FL::Q;
     constexpr int NP = npairs<DIM>();
-    constexpr bool F64 = std::is_same<R, double>::value;
     R m[Q], me[Q], src[Q];
     R ps[NP], pd[NP];
-    if constexpr (F64) {
+    if constexpr (EXACT) {
         static_for<0, Q>([&](auto Rr) {
             constexpr int r = decltype(Rr)::value;
             R acc = R(0);
@@ -237,7 +243,7 @@ __device__ __forceinline__ void collide_flow(const R *f, R base, R rho, R u, R v
         src[13] = 0; src[14] = a.c3_14 * (v * Fb); src[15] = a.c3_15 * (u * Fb);
         src[16] = 0; src[17] = 0; src[18] = 0;
     }
-    if constexpr (std::is_same<R, double>::value) {
+    if constexpr (EXACT) {
         // FP64: m_post = m - S(m - m_eq) + src dt, f_post = M^-1 m_post, in
         // the operation order of the FP64 oracle.
         static_for<0, Q>([&](auto Rr) {
@@ -258,7 +264,7 @@ __device__ __forceinline__ void collide_flow(const R *f, R base, R rho, R u, R v
             fp[al] = acc;
         });
     } else {
-        // FP32: the same update written as an increment,
+        // the same update written as an increment,
         // f_post = f + M^-1 (-S(m - m_eq) + src dt).  The conserved moments
         // have a zero increment, so mass and momentum pass through exactly
         // instead of being scaled every step by the rounded 1/|row|^2.
@@ -302,9 +308,9 @@ __device__ __forceinline__ void collide_flow(const R *f, R base, R rho, R u, R v
 
 // n = N g, relax towards n_eq, g_post = N^-1 n.  baseT = T (plain) or
 // T - Tref (shifted).  (2dim_mono.f90:522-569, 3dim_mono.f90:671-730)
-template <int DIM, typename R>
+template <int DIM, typename R, bool EXACT, typename A>
 __device__ __forceinline__ void collide_heat(const R *g, R baseT, R T, R u, R v,
-                                             R w, const StepArgs<R> &a, R *gp) {
+                                             R w, const A &a, R *gp) {
     using HT = Heat<DIM>;
     constexpr int Q = HT::Q;
     R nm[Q], ne[Q], q[Q];
@@ -341,7 +347,7 @@ __device__ __forceinline__ void collide_heat(const R *g, R baseT, R T, R u, R v,
         q[5] = a.q_e;
         q[6] = a.q_e;
     }
-    if constexpr (std::is_same<R, double>::value) {
+    if constexpr (EXACT) {
         static_for<0, Q>([&](auto Rr) {
             constexpr int r = decltype(Rr)::value;
             nm[r] = nm[r] - q[r] * (nm[r] - ne[r]);
@@ -390,8 +396,9 @@ template <typename R> struct AxisStencil {
     R wp[3], wm[3]; // weights for c = +1 and c = -1
 };
 
-template <typename R>
-__device__ __forceinline__ AxisStencil<R> load_stencil(const StepArgs<R> &a, int idx) {
+template <typename A>
+__device__ __forceinline__ AxisStencil<typename A::real> load_stencil(const A &a, int idx) {
+    using R = typename A::real;
     AxisStencil<R> s;
     int2 o = a.off[idx];
     s.o0 = o.x;
@@ -409,9 +416,9 @@ __device__ __forceinline__ AxisStencil<R> load_stencil(const StepArgs<R> &a, int
 // f1 + w0 (f0 - f1) + w2 (f2 - f1) (the weights sum to one), so that the
 // rounding of the stored weights acts on differences between neighbours
 // only and a uniform field streams unchanged.
-template <typename R>
+template <bool EXACT, typename R>
 __device__ __forceinline__ R lerp3(const R *w, R f0, R f1, R f2) {
-    if constexpr (std::is_same<R, double>::value) return w[0] * f0 + w[1] * f1 + w[2] * f2;
+    if constexpr (EXACT) return w[0] * f0 + w[1] * f1 + w[2] * f2;
     else return f1 + w[0] * (f0 - f1) + w[2] * (f2 - f1);
 }
 
@@ -432,8 +439,8 @@ __device__ __forceinline__ void stp(R *__restrict__ base, uint32_t idx, R v) {
 // One population: a 1D interpolation when one velocity component is
 // non-zero, a two-stage tensor interpolation (first along the higher axis,
 // then along the lower one) when two are.
-template <int CX, int CY, int CZ, typename R>
-__device__ __forceinline__ R interp_pop(const R *__restrict__ F, uint32_t node, R fc,
+template <int CX, int CY, int CZ, bool EXACT, typename S, typename R>
+__device__ __forceinline__ R interp_pop(const S *__restrict__ F, uint32_t node, R fc,
                                         const AxisStencil<R> *st, const uint32_t *stride) {
     constexpr int c[3] = {CX, CY, CZ};
     constexpr int nz = (CX != 0) + (CY != 0) + (CZ != 0);
@@ -441,8 +448,8 @@ __device__ __forceinline__ R interp_pop(const R *__restrict__ F, uint32_t node, 
         constexpr int ax = CX ? 0 : (CY ? 1 : 2);
         const AxisStencil<R> &s = st[ax];
         const R *w = c[ax] > 0 ? s.wp : s.wm;
-        return lerp3(w, ldp(F, node + (uint32_t)s.o0 * stride[ax]), fc,
-                     ldp(F, node + (uint32_t)s.o2 * stride[ax]));
+        return lerp3<EXACT>(w, R(ldp(F, node + (uint32_t)s.o0 * stride[ax])), fc,
+                            R(ldp(F, node + (uint32_t)s.o2 * stride[ax])));
     } else {
         constexpr int q = CX ? 0 : 1;          // second (outer) stage axis
         constexpr int p = CZ ? 2 : 1;          // first (inner) stage axis
@@ -452,10 +459,10 @@ __device__ __forceinline__ R interp_pop(const R *__restrict__ F, uint32_t node, 
         const uint32_t dp0 = (uint32_t)sp.o0 * stride[p], dp2 = (uint32_t)sp.o2 * stride[p];
         const uint32_t n0 = node + (uint32_t)sq.o0 * stride[q];
         const uint32_t n2 = node + (uint32_t)sq.o2 * stride[q];
-        R c0 = lerp3(wpp, ldp(F, n0 + dp0), ldp(F, n0), ldp(F, n0 + dp2));
-        R c1 = lerp3(wpp, ldp(F, node + dp0), fc, ldp(F, node + dp2));
-        R c2 = lerp3(wpp, ldp(F, n2 + dp0), ldp(F, n2), ldp(F, n2 + dp2));
-        return lerp3(wqq, c0, c1, c2);
+        R c0 = lerp3<EXACT>(wpp, R(ldp(F, n0 + dp0)), R(ldp(F, n0)), R(ldp(F, n0 + dp2)));
+        R c1 = lerp3<EXACT>(wpp, R(ldp(F, node + dp0)), fc, R(ldp(F, node + dp2)));
+        R c2 = lerp3<EXACT>(wpp, R(ldp(F, n2 + dp0)), R(ldp(F, n2)), R(ldp(F, n2 + dp2)));
+        return lerp3<EXACT>(wqq, c0, c1, c2);
     }
 }
 
@@ -497,9 +504,10 @@ __device__ __forceinline__ void tile_coords(int P, int G, int gx, int gy, int re
 // WPE > 0 asks the compiler for at least WPE resident waves per SIMD (it
 // then caps the registers per lane), trading load hoisting for latency
 // hiding; the autotuner picks.
-template <int DIM, typename R, bool SHIFT, bool MACRO, int WPE = 0>
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, int WPE = 0>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
-void k_step(StepArgs<R> a) {
+void k_step(StepArgs<R, S> a) {
+    constexpr bool EXACT = std::is_same<S, double>::value;
     using FL = Flow<DIM>;
     using HT = Heat<DIM>;
     constexpr int QF = FL::Q, QG = HT::Q;
@@ -522,11 +530,11 @@ void k_step(StepArgs<R> a) {
     R pcf[QF], pcg[QG], f[QF], g[QG];
     static_for<0, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        pcf[al] = ldp(a.pin + al * a.plane, node);
+        pcf[al] = R(ldp(a.pin + al * a.plane, node));
     });
     static_for<0, QG>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        pcg[al] = ldp(a.pin + (QF + al) * a.plane, node);
+        pcg[al] = R(ldp(a.pin + (QF + al) * a.plane, node));
     });
     R sp_f = R(0), sp_g = R(0);
     static_for<0, QF>([&](auto A) { sp_f += pcf[decltype(A)::value]; });
@@ -537,16 +545,16 @@ void k_step(StepArgs<R> a) {
     g[0] = pcg[0];
     static_for<1, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        f[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2]>(
+        f[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2], EXACT>(
             a.pin + al * a.plane, node, pcf[al], st, stride);
     });
     static_for<1, QG>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        g[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2]>(
+        g[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2], EXACT>(
             a.pin + (QF + al) * a.plane, node, pcg[al], st, stride);
         if constexpr (SHIFT && FL::c[al][HAX] != 0) {
-            constexpr double wc = HT::w[al] * FL::c[al][HAX];
-            g[al] += R(wc) * a.tcorr;
+            constexpr double wa = HT::w[al];
+            g[al] += R(wa) * a.tcorr[2 * idx[HAX] + (FL::c[al][HAX] > 0 ? 0 : 1)];
         }
     });
     static_for<0, DIM>([&](auto AX) {
@@ -584,7 +592,7 @@ void k_step(StepArgs<R> a) {
     // 2dim_mono.f90:47); the FP32 path recovers it from the conserved
     // zeroth moments of the node's own P(k).
     R F_prev;
-    if constexpr (std::is_same<R, double>::value) {
+    if constexpr (EXACT) {
         F_prev = ldp(a.force, node);
     } else {
         const R rho_prev = SHIFT ? R(1) + sp_f : sp_f;
@@ -593,7 +601,7 @@ void k_step(StepArgs<R> a) {
     }
 
     R srho = R(0), jx = R(0), jy = R(0), jz = R(0), sT = R(0);
-    if constexpr (std::is_same<R, double>::value) {
+    if constexpr (EXACT) {
         static_for<0, QF>([&](auto A) { srho += f[decltype(A)::value]; });
         static_for<1, QF>([&](auto A) {
             constexpr int al = decltype(A)::value;
@@ -624,7 +632,7 @@ void k_step(StepArgs<R> a) {
     const R rho = SHIFT ? R(1) + srho : srho;
     const R T = SHIFT ? tloc + sT : sT;
     R ux, uy, uz;
-    if constexpr (std::is_same<R, double>::value) {
+    if constexpr (EXACT) {
         if constexpr (DIM == 2) {
             ux = jx / rho;
             uy = (jy + a.half_dt * F_prev) / rho;
@@ -664,25 +672,25 @@ void k_step(StepArgs<R> a) {
 
     // 4. collision with the force of this step
     const R Fb = a.gbeta * T * rho;
-    if constexpr (std::is_same<R, double>::value) stp(a.force, node, Fb);
+    if constexpr (EXACT) stp(a.force, node, Fb);
     R fp[QF], gp[QG];
-    collide_flow<DIM, R, SHIFT>(f, SHIFT ? srho : rho, rho, ux, uy, uz, Fb, a, fp);
-    collide_heat<DIM, R>(g, SHIFT ? sT : T, T, ux, uy, uz, a, gp);
+    collide_flow<DIM, R, EXACT>(f, SHIFT ? srho : rho, rho, ux, uy, uz, Fb, a, fp);
+    collide_heat<DIM, R, EXACT>(g, SHIFT ? sT : T, T, ux, uy, uz, a, gp);
     static_for<0, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        stp(a.pout + al * a.plane, node, fp[al]);
+        stp(a.pout + al * a.plane, node, S(fp[al]));
     });
     static_for<0, QG>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        stp(a.pout + (QF + al) * a.plane, node, gp[al]);
+        stp(a.pout + (QF + al) * a.plane, node, S(gp[al]));
     });
 }
 
 // Initial collision (iteration 1's collision_U/collision_T): equilibrium
 // populations from the FP64 macroscopic fields (2dim_mono.f90:227-240),
 // collided in FP64 with F = g beta T rho, stored as R (shifted if SHIFT).
-template <int DIM, typename R, bool SHIFT>
-__global__ __launch_bounds__(256) void k_prologue(StepArgs<R> a, StepArgs<double> ad) {
+template <int DIM, typename S, typename R, bool SHIFT>
+__global__ __launch_bounds__(256) void k_prologue(StepArgs<R, S> a, StepArgs<double> ad) {
     using FL = Flow<DIM>;
     using HT = Heat<DIM>;
     constexpr int QF = FL::Q, QG = HT::Q;
@@ -708,20 +716,20 @@ __global__ __launch_bounds__(256) void k_prologue(StepArgs<R> a, StepArgs<double
         g[al] = wa * T * (1.0 + HT::cs2inv * un);
     });
     const double Fb = ad.gbeta * T * rho;
-    if constexpr (std::is_same<R, double>::value) a.force[node] = Fb;
-    collide_flow<DIM, double, false>(f, rho, rho, u, v, w, Fb, ad, fp);
-    collide_heat<DIM, double>(g, T, T, u, v, w, ad, gp);
+    if constexpr (std::is_same<S, double>::value) a.force[node] = Fb;
+    collide_flow<DIM, double, true>(f, rho, rho, u, v, w, Fb, ad, fp);
+    collide_heat<DIM, double, true>(g, T, T, u, v, w, ad, gp);
     static_for<0, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
         constexpr double wa = FL::w[al];
-        a.pout[al * a.plane + node] = R(SHIFT ? fp[al] - wa : fp[al]);
+        a.pout[al * a.plane + node] = S(SHIFT ? fp[al] - wa : fp[al]);
     });
     int hidx = DIM == 2 ? (int)(node % a.n) : (int)((node / a.n) % a.n);
     const double tloc = SHIFT ? double(a.tref[hidx]) : 0.0;
     static_for<0, QG>([&](auto A) {
         constexpr int al = decltype(A)::value;
         constexpr double wa = HT::w[al];
-        a.pout[(QF + al) * a.plane + node] = R(SHIFT ? gp[al] - wa * tloc : gp[al]);
+        a.pout[(QF + al) * a.plane + node] = S(SHIFT ? gp[al] - wa * tloc : gp[al]);
     });
 }
 

@@ -16,11 +16,11 @@ hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a0, cons
     a.remap = (ls.remap == 2 && a.gy % 8) ? 0 : ls.remap;
     dim3 grid(a.ntiles);
     if (dim == 2) {
-        if (macro) hipLaunchKernelGGL((k_step<2, double, false, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step<2, double, false, false>), grid, block, 0, s, a);
+        if (macro) hipLaunchKernelGGL((k_step<2, double, double, false, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_step<2, double, double, false, false>), grid, block, 0, s, a);
     } else {
-        if (macro) hipLaunchKernelGGL((k_step<3, double, false, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step<3, double, false, false>), grid, block, 0, s, a);
+        if (macro) hipLaunchKernelGGL((k_step<3, double, double, false, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_step<3, double, double, false, false>), grid, block, 0, s, a);
     }
     return hipGetLastError();
 }
@@ -30,8 +30,8 @@ hipError_t launch_prologue_f64(int dim, const StepArgs<double> &a,
     long nn = dim == 2 ? (long)a.n * a.n : (long)a.n * a.n * a.n;
     dim3 block(256);
     dim3 grid((unsigned)((nn + 255) / 256));
-    if (dim == 2) hipLaunchKernelGGL((k_prologue<2, double, false>), grid, block, 0, s, a, ad);
-    else hipLaunchKernelGGL((k_prologue<3, double, false>), grid, block, 0, s, a, ad);
+    if (dim == 2) hipLaunchKernelGGL((k_prologue<2, double, double, false>), grid, block, 0, s, a, ad);
+    else hipLaunchKernelGGL((k_prologue<3, double, double, false>), grid, block, 0, s, a, ad);
     return hipGetLastError();
 }
 

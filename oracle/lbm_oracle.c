@@ -615,7 +615,8 @@ void orc_bounceback_t(orc *s) {
 
 /* ---------------------------------------------------------------- macros */
 /* macro_u (2M:726-739, 3M:988-1010): rho = sum f; u = (sum c f + F dt/2)/rho
- * with the force stored by this step's collision. */
+ * with the force stored by this step's collision.  In 3D only w carries the
+ * force term (3M:1000-1005). */
 void orc_macro_u(orc *s) {
     const int q = s->qf;
     const double dt = s->p[ORC_P_DT];
@@ -630,9 +631,15 @@ void orc_macro_u(orc *s) {
             if (s->c[a][2]) jz += s->c[a][2] > 0 ? f[a] : -f[a];
         }
         s->rho[nd] = rho;
-        s->u[nd] = (jx + 0.5 * dt * s->Fx[nd]) / rho;
-        s->v[nd] = (jy + 0.5 * dt * s->Fy[nd]) / rho;
-        s->w[nd] = s->dim == 3 ? (jz + 0.5 * dt * s->Fz[nd]) / rho : 0.0;
+        if (s->dim == 2) {
+            s->u[nd] = (jx + 0.5 * dt * s->Fx[nd]) / rho;
+            s->v[nd] = (jy + 0.5 * dt * s->Fy[nd]) / rho;
+            s->w[nd] = 0.0;
+        } else {
+            s->u[nd] = jx / rho;
+            s->v[nd] = jy / rho;
+            s->w[nd] = (jz + 0.5 * dt * s->Fz[nd]) / rho;
+        }
     }
 }
 

@@ -213,42 +213,6 @@ def test_large_3d_fp32_vs_fp64_gpu(hip):
     assert abs(na[0] - nb[0]) <= NU_TOL * abs(nb[0])
 
 
-ZSHAPES = [(64, 4), (64, 8), (128, 2), (128, 4), (32, 8), (96, 6)]
-
-
-@pytest.mark.parametrize("n,zchunk", [(22, 7), (22, 64), (70, 16)])
-def test_zmarch_kernel_matches_node_kernel_and_oracle(oracle_mod, hip, n, zchunk):
-    """Every z-marching tile shape against the node-per-thread FP32 kernel
-    (same arithmetic, different interpolation-stage order: <= 1e-6) and the
-    FP64 oracle (FP32 tolerance), including partial tiles and chunks."""
-    print(f"\n[z-march] n={n} zchunk={zchunk}", flush=True)
-    steps = 60
-    o = oracle_mod.Oracle(3, n, 1e5)
-    o.init_rest()
-    o.step(steps)
-    ref = o.fields()
-    base = hip.Solver(3, n, 1e5)
-    base.set_launch(64, 4, remap=1, kind=0)
-    base.init_rest()
-    base.step(steps)
-    node_fields = base.fields()
-    base.close()
-    for bx, by in ZSHAPES:
-        s = hip.Solver(3, n, 1e5)
-        s.set_launch(bx, by, remap=1, kind=1, zchunk=zchunk)
-        assert s.get_launch()["kind"] == 1, (bx, by)
-        s.init_rest()
-        s.step(steps)
-        got = s.fields()
-        e_node = field_errors(got, node_fields)
-        e_ref = field_errors(got, ref)
-        print(f"  shape=({bx},{by}) vs node={e_node} vs oracle={e_ref}", flush=True)
-        for k in ("u", "v", "w", "T", "rho"):
-            assert e_node[k] <= 1e-6, (bx, by, k, e_node[k])
-            assert e_ref[k] <= FP32_TOL, (bx, by, k, e_ref[k])
-        s.close()
-
-
 def test_autotune_restores_state(hip):
     print("\n[autotune] 3D 40", flush=True)
     s = hip.Solver(3, 40, 1e5)
@@ -279,7 +243,7 @@ def test_tile_orders_give_identical_results(hip, dim, n, bx, by):
     outs = []
     for remap in (0, 1, 2):
         s = hip.Solver(dim, n, 1e5)
-        s.set_launch(bx, by, remap=remap, kind=0)
+        s.set_launch(bx, by, remap=remap)
         s.init_rest()
         s.step(25)
         outs.append(s.fields())

@@ -1,0 +1,418 @@
+"""Pins of the FP64 CPU oracle (oracle/lbm_oracle.c) -- CPU only.
+
+The reference has no tests or golden vectors, and it cannot be built here
+(it needs an MPI library module).  The oracle is pinned instead by
+  * outputs of the reference's own runs: its printed parameters, its
+    residuals at itc = 20000, the iterations at which its residuals first
+    reach 1e-6 and 1e-8 on the 129^2 cavity (392 000 and 536 000);
+  * the reference's linear tables as coded, extracted from its source text
+    by tests/golden/extract_reference_tables.py (moment matrices and
+    inverses, relaxation-rate assignments, wall rules);
+  * closed forms the mathematics fixes: textbook equilibria and Guo forcing
+    in moment space, exactness of quadratic interpolation, conservation,
+    fixed points, the conduction Nusselt number;
+chosen so that a dropped term, a wrong sign, index or stencil offset fails
+one of them.
+"""
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLES = json.load(open(os.path.join(HERE, "golden", "reference_tables.json")))
+
+C2 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0],
+               [1, 1, 0], [-1, 1, 0], [-1, -1, 0], [1, -1, 0]])
+C3 = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1],
+               [0, 0, -1], [1, 1, 0], [-1, 1, 0], [1, -1, 0], [-1, -1, 0], [1, 0, 1],
+               [-1, 0, 1], [1, 0, -1], [-1, 0, -1], [0, 1, 1], [0, -1, 1],
+               [0, 1, -1], [0, -1, -1]])
+W2 = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+W3 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+WT2 = np.array([0.5] + [0.125] * 4)
+WT3 = np.array([0.5] + [1 / 12] * 6)
+
+
+def frac_matrix(rows):
+    return [[Fraction(x) for x in r] for r in rows]
+
+
+def as_float(rows):
+    return np.array([[float(Fraction(x)) for x in r] for r in rows])
+
+
+def lattice(dim):
+    return (C2, W2, WT2, 4.0) if dim == 2 else (C3, W3, WT3, 6.0)
+
+
+# ------------------------------------------------------------- grid, params
+def test_parameters_match_reference_printout(oracle_mod):
+    # The reference prints real(length_LB), real(viscosity_LB) and real(tauf)
+    # for the 129^2, Ra = 1e4 cavity: 874.25805, 0.42482606, 1.7744782.
+    o = oracle_mod.Oracle(2, 129, 1e4)
+    p = o.params()
+    assert abs(p["L_LB"] - 874.25805) < 1e-4
+    assert abs(p["nu"] - 0.42482606) < 1e-7
+    assert abs(p["tauf"] - 1.7744782) < 1e-6
+
+
+@pytest.mark.parametrize("n", [5, 33, 65, 129, 256, 257, 513, 1025])
+def test_grid_closed_form(oracle_mod, n):
+    A = 3.2
+    o = oracle_mod.Oracle(2, n, 1e5, A=A)
+    p = o.params()
+    x = o.coords()
+    X = [0.5 * (math.erf(A * (i / (n + 1) - 0.5)) / math.erf(0.5 * A) + 1.0)
+         for i in range(n + 2)]
+    dx0 = X[1] - X[0]
+    assert p["dx0"] == dx0
+    assert p["L_LB"] == 1.0 / dx0
+    assert p["dt"] == 1.0                      # dx0 * (1/dx0) rounds to 1
+    ref = np.array([(X[i] - dx0 / 2.0) * (1.0 / dx0) for i in range(1, n + 1)])
+    assert np.array_equal(x, ref)
+    H = p["H"]
+    assert abs(x[0] - 0.5) < 1e-12              # half a unit from the wall at 0
+    assert abs(x[-1] + 0.5 - H) < 1e-9 * H       # and from the wall at H
+    assert np.all(np.diff(x) > 0)
+    assert np.abs(x + x[::-1] - H).max() < 1e-9 * H   # symmetric stretching
+
+
+def test_stencils_are_one_sided_at_walls(oracle_mod):
+    n = 9
+    st = oracle_mod.Oracle(2, n, 1e5).stencil()
+    assert list(st[0]) == [1, 0, 2]
+    assert list(st[n - 1]) == [n - 2, n - 1, n - 3]
+    for i in range(1, n - 1):
+        assert list(st[i]) == [i - 1, i, i + 1]
+
+
+@pytest.mark.parametrize("dim,n,Ra,Pr", [(2, 129, 1e4, 0.71), (2, 65, 1e6, 7.0),
+                                         (3, 33, 1e5, 0.71), (3, 17, 1e3, 2.0)])
+def test_parameter_relations(oracle_mod, dim, n, Ra, Pr):
+    p = oracle_mod.Oracle(dim, n, Ra, Pr=Pr).params()
+    H = p["H"]
+    assert abs(p["nu"] - 0.1 * H * math.sqrt(Pr) / math.sqrt(3 * Ra)) < 1e-15 * p["nu"] * 10
+    assert p["kappa"] == p["nu"] / Pr
+    assert abs(p["gbeta"] * H ** 3 - Ra * p["nu"] * p["kappa"]) < 1e-12 * Ra * p["nu"] * p["kappa"]
+    assert abs(p["tauf"] - (3 * p["nu"] + 0.5)) < 1e-15 * 10
+    # S_q from S_nu fixes (1/S_nu - 1/2)(1/S_q - 1/2) = 3/16 (the value that
+    # places half-way bounce-back walls exactly half a link out)
+    lam = (1 / p["Snu"] - 0.5) * (1 / p["Sq"] - 0.5)
+    assert abs(lam - 3 / 16) < 1e-14
+    # sigma_k gives the thermal diffusivity kappa on the D2Q5 (cs^2 = 1/4)
+    # or D3Q7 (cs^2 = 1/6) lattice with rest weight 1/2
+    cs2 = 0.25 if dim == 2 else 1 / 6
+    assert abs((1 / p["sigk"] - 0.5) * cs2 - p["kappa"]) < 1e-12 * p["kappa"]
+
+
+# -------------------------------------------------- reference tables sanity
+@pytest.mark.parametrize("dim", ["2d", "3d"])
+def test_reference_tables_are_inverse_pairs(dim):
+    t = TABLES[dim]
+    for a, b in (("M", "Minv"), ("N", "Ninv")):
+        M, Mi = frac_matrix(t[a]), frac_matrix(t[b])
+        q = len(M)
+        for r in range(q):
+            for c in range(q):
+                s = sum(Mi[r][k] * M[k][c] for k in range(q))
+                assert s == (1 if r == c else 0), (dim, a, r, c)
+
+
+# --------------------------------------------------------------- collision
+def _setup_random_state(o, dim, seed):
+    rng = np.random.default_rng(seed)
+    C, W, WT, cs = lattice(dim)
+    shape = o.shape
+    rho = 1.0 + 0.01 * rng.standard_normal(shape)
+    u = [0.02 * rng.standard_normal(shape) for _ in range(3)]
+    if dim == 2:
+        u[2][...] = 0.0
+    T = 0.5 + 0.3 * rng.standard_normal(shape)
+    f = o.array("f")
+    g = o.array("g")
+    cu = sum(u[d][..., None] * C[None, :, d] if dim == 3 else u[d][..., None, ] * C[:, d]
+             for d in range(3))
+    u2 = sum(x * x for x in u)[..., None]
+    feq = rho[..., None] * W * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * u2)
+    f[...] = feq * (1 + 0.05 * rng.standard_normal(f.shape))
+    g[...] = (WT * T[..., None] * (1 + cs * cu[..., :len(WT)])) * \
+        (1 + 0.05 * rng.standard_normal(g.shape))
+    for name, val in (("rho", rho), ("u", u[0]), ("v", u[1]), ("w", u[2]), ("T", T)):
+        o.array(name)[...] = val
+    return rho, u, T
+
+
+@pytest.mark.parametrize("dim,seed", [(2, 1), (3, 2)])
+def test_flow_collision_matches_reference_moment_space(oracle_mod, dim, seed):
+    """f_post = M^-1 [m - S(m - m_eq) + (I - S/2) M F_Guo dt] with the
+    reference's M, M^-1 and S assignment, the textbook second-order
+    equilibrium (m_eq = M f_eq) and the textbook Guo forcing term."""
+    n = 5
+    o = oracle_mod.Oracle(dim, n, 1e5)
+    rho, u, T = _setup_random_state(o, dim, seed)
+    p = o.params()
+    t = TABLES["2d" if dim == 2 else "3d"]
+    M, Minv = as_float(t["M"]), as_float(t["Minv"])
+    S = np.array([{"Snu": p["Snu"], "Sq": p["Sq"]}.get(s, None) or float(s) for s in t["s"]])
+    C, W, _, _ = lattice(dim)
+    f0 = o.array("f").copy()
+    o.collision_u()
+    fpost = o.array("fpost")
+    gax = 1 if dim == 2 else 2            # gravity along +y (2D) / +z (3D)
+    for idx in np.ndindex(o.shape):
+        uu = np.array([u[0][idx], u[1][idx], u[2][idx]])
+        F = np.zeros(3)
+        F[gax] = p["gbeta"] * T[idx] * rho[idx]
+        cu = C @ uu
+        feq = rho[idx] * W * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * uu @ uu)
+        guo = W * (3 * (C @ F - uu @ F) + 9 * cu * (C @ F))
+        m = M @ f0[idx]
+        mp = m - S * (m - M @ feq) + (1 - S / 2) * (M @ guo) * p["dt"]
+        ref = Minv @ mp
+        assert np.abs(fpost[idx] - ref).max() < 1e-13 * np.abs(ref).max(), idx
+    # the rest population does not stream: f0 <- f_post0
+    assert np.array_equal(o.array("f")[..., 0], fpost[..., 0])
+
+
+@pytest.mark.parametrize("dim,seed", [(2, 3), (3, 4)])
+def test_thermal_collision_matches_reference_moment_space(oracle_mod, dim, seed):
+    n = 5
+    o = oracle_mod.Oracle(dim, n, 1e5)
+    rho, u, T = _setup_random_state(o, dim, seed)
+    p = o.params()
+    t = TABLES["2d" if dim == 2 else "3d"]
+    N, Ninv = as_float(t["N"]), as_float(t["Ninv"])
+    Q = np.array([p["sigk"] if s == "sig_k" else float(s) for s in t["Q"]])
+    C, _, WT, cs = lattice(dim)
+    Cg = C[:len(WT)]
+    g0 = o.array("g").copy()
+    o.collision_t()
+    gpost = o.array("gpost")
+    for idx in np.ndindex(o.shape):
+        uu = np.array([u[0][idx], u[1][idx], u[2][idx]])
+        geq = WT * T[idx] * (1 + cs * (Cg @ uu))
+        nm = N @ g0[idx]
+        ref = Ninv @ (nm - Q * (nm - N @ geq))
+        assert np.abs(gpost[idx] - ref).max() < 1e-13 * np.abs(ref).max(), idx
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_collision_conserves_mass_and_adds_force(oracle_mod, dim):
+    o = oracle_mod.Oracle(dim, 6, 1e5)
+    rho, u, T = _setup_random_state(o, dim, 7)
+    C, _, WT, _ = lattice(dim)
+    f0, g0 = o.array("f").copy(), o.array("g").copy()
+    o.collision_u()
+    o.collision_t()
+    fp, gp = o.array("fpost"), o.array("gpost")
+    p = o.params()
+    assert np.abs(fp.sum(-1) - f0.sum(-1)).max() < 1e-14
+    # Q_0 = 1: the zeroth thermal moment is reset to the temperature field
+    # (equal to sum g in the time loop, where T comes from macro_t)
+    assert np.abs(gp.sum(-1) - T).max() < 1e-14
+    gax = 1 if dim == 2 else 2
+    Fb = p["gbeta"] * T * rho
+    for d in range(dim):
+        dj = (fp * C[:, d]).sum(-1) - (f0 * C[:, d]).sum(-1)
+        expect = Fb * p["dt"] if d == gax else 0.0
+        assert np.abs(dj - expect).max() < 1e-15
+
+
+# ------------------------------------------------------------ interpolation
+def test_interpolateF_is_exact_for_quadratics(oracle_mod):
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        a, b, c = rng.standard_normal(3)
+        xs = np.sort(rng.uniform(-5, 5, 3))
+        x = rng.uniform(-6, 6)
+        q = lambda t: a + b * t + c * t * t
+        got = oracle_mod.interpolateF(xs[0], xs[1], xs[2], x, q(xs[0]), q(xs[1]), q(xs[2]))
+        assert abs(got - q(x)) < 1e-11 * (1 + abs(q(x)))
+    # zero offset reproduces the middle value exactly
+    assert oracle_mod.interpolateF(0.1, 0.7, 2.5, 0.7, 3.0, -1.25, 9.0) == -1.25
+
+
+@pytest.mark.parametrize("dim,n,A", [(2, 9, 3.2), (2, 12, 1.5), (3, 7, 3.2)])
+def test_streaming_is_exact_for_quadratic_fields(oracle_mod, dim, n, A):
+    """With post-collision fields that are (bi/tri)quadratic in the node
+    coordinates, interpolation-supplemented streaming must return exactly
+    f(a, x) = f_post(a, x - c_a dt) at every node, walls included (one-sided
+    stencils are exact for quadratics): pins the shift direction, the
+    stencils and the interpolation planes of every direction."""
+    o = oracle_mod.Oracle(dim, n, 1e5, A=A)
+    x = o.coords()
+    dt = o.params()["dt"]
+    C, _, WT, _ = lattice(dim)
+    rng = np.random.default_rng(5)
+    coef = rng.standard_normal((len(C), 3, 3))
+
+    def q(alpha, shift):
+        vals = []
+        for d in range(dim):
+            t = x - shift[d]
+            vals.append(coef[alpha, d, 0] + coef[alpha, d, 1] * t / x[-1] +
+                        coef[alpha, d, 2] * (t / x[-1]) ** 2)
+        if dim == 2:
+            return vals[1][:, None] * vals[0][None, :]
+        return vals[2][:, None, None] * vals[1][None, :, None] * vals[0][None, None, :]
+
+    fpost, gpost = o.array("fpost"), o.array("gpost")
+    for a in range(len(C)):
+        fpost[..., a] = q(a, (0, 0, 0))
+    for a in range(len(WT)):
+        gpost[..., a] = q(a, (0, 0, 0)) + 0.5
+    o.interpolate()
+    f, g = o.array("f"), o.array("g")
+    for a in range(1, len(C)):
+        ref = q(a, C[a] * dt)
+        assert np.abs(f[..., a] - ref).max() < 1e-9 * np.abs(ref).max(), a
+    for a in range(1, len(WT)):
+        ref = q(a, C[a] * dt) + 0.5
+        assert np.abs(g[..., a] - ref).max() < 1e-9 * np.abs(ref).max(), a
+
+
+# -------------------------------------------------------------------- walls
+def _node_set(expr, n):
+    e = [s.strip() for s in expr]
+    out = []
+    for s in e:
+        if s == "1":
+            out.append([0])
+        elif s in ("nx", "ny", "nz", "nyLocal", "nzLocal"):
+            out.append([n - 1])
+        else:
+            out.append(list(range(n)))
+    return out
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_wall_rules_match_reference(oracle_mod, dim):
+    n = 6
+    o = oracle_mod.Oracle(dim, n, 1e5, Thot=1.25, Tcold=-0.5)
+    t = TABLES["2d" if dim == 2 else "3d"]
+    rng = np.random.default_rng(9)
+    f, fpost = o.array("f"), o.array("fpost")
+    g, gpost = o.array("g"), o.array("gpost")
+    f[...] = rng.standard_normal(f.shape)
+    fpost[...] = rng.standard_normal(fpost.shape)
+    g[...] = rng.standard_normal(g.shape)
+    gpost[...] = rng.standard_normal(gpost.shape)
+    f0, g0 = f.copy(), g.copy()
+    o.bounceback_u()
+    o.bounceback_t()
+    _, _, WT, _ = lattice(dim)
+    expect_f, expect_g = f0.copy(), g0.copy()
+    for rule in t["walls_u"]:
+        axes = _node_set(rule["node"], n)   # Fortran order (i, j[, k])
+        for ijk in np.ndindex(*[len(a) for a in axes]):
+            node = tuple(axes[d][ijk[d]] for d in range(dim))[::-1]
+            expect_f[node + (rule["to"],)] = fpost[node + (rule["from"],)]
+    temps = {"Thot": 1.25, "Tcold": -0.5}
+    for rule in t["walls_T"]:
+        axes = _node_set(rule["node"], n)
+        for ijk in np.ndindex(*[len(a) for a in axes]):
+            node = tuple(axes[d][ijk[d]] for d in range(dim))[::-1]
+            v = rule["sign"] * gpost[node + (rule["from"],)]
+            if rule["wall_temperature"]:
+                v += 2.0 * WT[rule["to"]] * temps[rule["wall_temperature"]]
+            expect_g[node + (rule["to"],)] = v
+    assert np.array_equal(f, expect_f)
+    assert np.array_equal(g, expect_g)
+
+
+# ------------------------------------------------------------------- macros
+@pytest.mark.parametrize("dim", [2, 3])
+def test_macros_closed_form(oracle_mod, dim):
+    o = oracle_mod.Oracle(dim, 5, 1e5)
+    rng = np.random.default_rng(3)
+    C, _, WT, _ = lattice(dim)
+    f, g = o.array("f"), o.array("g")
+    f[...] = rng.uniform(0.01, 0.2, f.shape)
+    g[...] = rng.uniform(0.01, 0.2, g.shape)
+    F = [o.array(k) for k in ("Fx", "Fy", "Fz")]
+    for a in F:
+        a[...] = 1e-4 * rng.standard_normal(a.shape)
+    if dim == 2:
+        F[2][...] = 0
+    o.macro_u()
+    o.macro_t()
+    rho = f.sum(-1)
+    assert np.allclose(o.array("rho"), rho, rtol=1e-15, atol=0)
+    for d, name in enumerate(("u", "v", "w")[:dim]):
+        # the force term enters every component in 2D, only w in 3D
+        force = F[d] if (dim == 2 or d == 2) else 0.0
+        ref = ((f * C[:, d]).sum(-1) + 0.5 * force) / rho
+        assert np.allclose(o.array(name), ref, rtol=1e-13, atol=1e-15)
+    assert np.allclose(o.array("T"), g.sum(-1), rtol=1e-15, atol=0)
+
+
+# ------------------------------------------------------ residual, Nusselt
+def test_check_is_windowed_relative_l2(oracle_mod):
+    o = oracle_mod.Oracle(2, 9, 1e5)
+    rng = np.random.default_rng(1)
+    u, v, T = (o.array(k) for k in ("u", "v", "T"))
+    u[...], v[...], T[...] = rng.standard_normal((3, 9, 9))
+    assert o.check() == (1.0, 1.0)      # the snapshot starts at zero
+    u0, v0, T0 = u.copy(), v.copy(), T.copy()
+    u *= 1.01
+    T += 0.02
+    eu, et = o.check()
+    ref_u = math.sqrt(((u - u0) ** 2 + (v - v0) ** 2).sum()) / math.sqrt((u ** 2 + v ** 2).sum())
+    ref_t = math.sqrt(((T - T0) ** 2).sum()) / math.sqrt((T ** 2).sum())
+    assert abs(eu - ref_u) < 1e-14 and abs(et - ref_t) < 1e-14
+
+
+@pytest.mark.parametrize("dim,n,A", [(2, 17, 3.2), (2, 33, 1.0), (3, 9, 3.2)])
+def test_nusselt_of_conduction_profile_is_one(oracle_mod, dim, n, A):
+    o = oracle_mod.Oracle(dim, n, 1e5, Thot=1.0, Tcold=0.0, A=A)
+    x = o.coords()
+    H = o.params()["H"]
+    prof = 1.0 - x / H
+    T = o.array("T")
+    if dim == 2:
+        T[...] = prof[None, :]
+    else:
+        T[...] = prof[None, :, None]
+    nh, nc = o.nusselt()
+    assert abs(nh - 1.0) < 1e-12 and abs(nc - 1.0) < 1e-12
+
+
+def test_rest_state_without_force_is_a_fixed_point(oracle_mod):
+    o = oracle_mod.Oracle(2, 11, 1e5, Thot=0.0, Tcold=0.0)
+    o.init_fields(None, None, None, np.zeros((11, 11)))
+    o.step(10)
+    rho, u, v, w, T = o.fields()
+    # only rounding noise: a spurious source would be many orders larger
+    assert np.abs(u).max() < 1e-14 and np.abs(v).max() < 1e-14
+    assert np.abs(rho - 1).max() < 1e-13 and np.abs(T).max() < 1e-16
+
+
+# ---------------------------------------------------- coupled trajectory
+@pytest.mark.slow
+def test_residuals_at_itc_20000_match_reference_run(oracle_mod):
+    """The reference run (129^2, Ra = 1e4, np = 2) printed errorU = 8.0e-2
+    and errorT = 1.06e-2 at itc = 20000, its itc_max."""
+    o = oracle_mod.Oracle(2, 129, 1e4)
+    itc, eu, et = o.run(1e-6, 1e-6, 20000)
+    assert itc == 20000
+    assert abs(eu - 8.0e-2) < 5e-4
+    assert abs(et - 1.06e-2) < 5e-5
+
+
+def test_steady_fixture_matches_reference_run():
+    """The committed oracle steady state (tests/golden/make_steady_129.py)
+    reached residuals <= 1e-6 first at itc = 392 000 and <= 1e-8 at
+    itc = 536 000, the iterations the reference run reported for the same
+    cavity; its wall Nusselt numbers agree with each other (energy balance)
+    and with de Vahl Davis' benchmark value 2.243 at Ra = 1e4."""
+    d = np.load(os.path.join(HERE, "golden", "steady_2d_129_ra1e4.npz"))
+    assert int(d["itc"]) == 536000
+    assert int(d["itc_1e6"]) == 392000
+    hist = d["history"]
+    assert hist[0, 1] == 1.0 and hist[0, 2] == 1.0
+    assert abs(float(d["nu_hot"]) - float(d["nu_cold"])) < 1e-6
+    assert abs(float(d["nu_hot"]) - 2.243) < 2e-3
