@@ -71,7 +71,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch", default="",
-                    help="fixed launch shape bx,by,remap,occupancy (implies no autotune)")
+                    help="fixed launch shape bx,by,remap,occupancy,variant (implies no autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (5 times), the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -96,8 +96,8 @@ def main():
     s = lbmhip.Solver(cfg["dim"], cfg["n"], cfg["Ra"], precision=prec, device=0)
     s.init_rest()
     if args.launch:
-        bx, by, remap, occ = (int(v) for v in args.launch.split(","))
-        s.set_launch(bx, by, remap=remap, occupancy=occ)
+        vals = [int(v) for v in args.launch.split(",")] + [0]
+        s.set_launch(vals[0], vals[1], remap=vals[2], occupancy=vals[3], variant=vals[4])
     elif not args.no_autotune:
         s.autotune(10)   # restores the state it started from
     if args.warmup:

@@ -177,9 +177,11 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes);
  *    y and sweeps it plane by plane (needs the number of y tiles divisible by
  *    8, else 0 is used).
  *  occupancy (3D, FP32 storage): 0, or 5 or 6 resident waves per SIMD, which
- *    caps the registers per lane. */
+ *    caps the registers per lane.
+ *  variant: step-kernel variant; 0 = one node per thread, every stencil
+ *    point read through the caches (the only variant of this build). */
 typedef struct lbm_launch {
-    int32_t bx, by, remap, occupancy;
+    int32_t bx, by, remap, occupancy, variant;
 } lbm_launch;
 int32_t lbm_set_launch(lbm_handle h, const lbm_launch *launch);
 int32_t lbm_get_launch(lbm_handle h, lbm_launch *launch);

@@ -23,21 +23,28 @@ __global__ void k_init_rest(double *rho, double *u, double *v, double *w, double
 
 template <int NQ>
 __device__ void block_reduce_store(double (&acc)[NQ], double *out) {
-    __shared__ double sh[NQ][kRedThreads];
-    const int t = threadIdx.x;
+    // butterfly over the 64 lanes of each wave (a fixed pairing, so every
+    // lane ends with the same, reproducible sum), then the wave sums in
+    // wave order
+    constexpr int NW = kRedThreads / 64;
+    __shared__ double sh[NQ][NW];
 #pragma unroll
-    for (int q = 0; q < NQ; q++) sh[q][t] = acc[q];
-    __syncthreads();
-    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-        if (t < s) {
+    for (int q = 0; q < NQ; q++)
 #pragma unroll
-            for (int q = 0; q < NQ; q++) sh[q][t] += sh[q][t + s];
-        }
-        __syncthreads();
+        for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) sh[q][wv] = acc[q];
     }
-    if (t == 0) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; q++) out[q] = sh[q][0];
+        for (int q = 0; q < NQ; q++) {
+            double s = sh[q][0];
+            for (int w = 1; w < NW; w++) s += sh[q][w];
+            out[q] = s;
+        }
     }
 }
 

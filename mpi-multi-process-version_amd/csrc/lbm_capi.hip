@@ -634,12 +634,14 @@ int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
     if (!h || !l) return fail(LBM_ERR_ARG, "null argument");
     if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
         l->remap < 0 || l->remap > 2 ||
-        (l->occupancy != 0 && l->occupancy != 5 && l->occupancy != 6))
+        (l->occupancy != 0 && l->occupancy != 5 && l->occupancy != 6) ||
+        l->variant != 0)
         return fail(LBM_ERR_ARG, "invalid launch configuration");
     h->shape.bx = l->bx;
     h->shape.by = l->by;
     h->shape.remap = l->remap;
     h->shape.occ = l->occupancy;
+    h->shape.variant = l->variant;
     return LBM_OK;
 }
 
@@ -649,6 +651,7 @@ int32_t lbm_get_launch(lbm_handle h, lbm_launch *l) {
     l->by = h->shape.by;
     l->remap = h->shape.remap;
     l->occupancy = (h->dim == 3 && h->p.precision == LBM_PREC_FP32_SHIFTED) ? h->shape.occ : 0;
+    l->variant = h->shape.variant;
     return LBM_OK;
 }
 
@@ -686,12 +689,14 @@ int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms) {
     const int nocc = (h->dim == 3 && h->p.precision == LBM_PREC_FP32_SHIFTED) ? 3 : 1;
     for (const auto &cd : cand) {
         if (cd[0] > 2 * h->n && cd[0] > 64) continue;
-        for (int combo = 0; combo < 3 * nocc; combo++) {
+        const int nvar = 1;
+        for (int combo = 0; combo < 3 * nocc * nvar; combo++) {
             static const int occs[3] = {0, 5, 6};
             h->shape.bx = cd[0];
             h->shape.by = cd[1];
             h->shape.remap = combo % 3;
-            h->shape.occ = occs[combo / 3];
+            h->shape.occ = occs[(combo / 3) % nocc];
+            h->shape.variant = combo / (3 * nocc);
             double ms = 0;
             // one untimed warm-up launch per candidate, then the timed trial
             if ((rc = do_steps(*h, 1)) || (rc = lbm_time_steps(h, steps_per_trial, &ms))) {

@@ -4,6 +4,15 @@
 
 namespace lbm {
 
+template <int DIM, int WPE>
+static void launch_k(bool macro, dim3 grid, dim3 block, const StepArgs<float> &a,
+                     hipStream_t s) {
+    if (macro)
+        hipLaunchKernelGGL((k_step<DIM, float, float, true, true, WPE>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_step<DIM, float, float, true, false, WPE>), grid, block, 0, s, a);
+}
+
 hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
                            hipStream_t s) {
     StepArgs<float> a = a0;
@@ -13,19 +22,10 @@ hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a0, const
     a.ntiles = a.gx * a.gy * (dim == 3 ? a.n : 1);
     a.remap = (ls.remap == 2 && a.gy % 8) ? 0 : ls.remap;
     dim3 grid(a.ntiles);
-    if (dim == 2) {
-        if (macro) hipLaunchKernelGGL((k_step<2, float, float, true, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step<2, float, float, true, false>), grid, block, 0, s, a);
-    } else if (ls.occ == 5) {
-        if (macro) hipLaunchKernelGGL((k_step<3, float, float, true, true, 5>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step<3, float, float, true, false, 5>), grid, block, 0, s, a);
-    } else if (ls.occ == 6) {
-        if (macro) hipLaunchKernelGGL((k_step<3, float, float, true, true, 6>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step<3, float, float, true, false, 6>), grid, block, 0, s, a);
-    } else {
-        if (macro) hipLaunchKernelGGL((k_step<3, float, float, true, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step<3, float, float, true, false>), grid, block, 0, s, a);
-    }
+    if (dim == 2) launch_k<2, 0>(macro, grid, block, a, s);
+    else if (ls.occ == 5) launch_k<3, 5>(macro, grid, block, a, s);
+    else if (ls.occ == 6) launch_k<3, 6>(macro, grid, block, a, s);
+    else launch_k<3, 0>(macro, grid, block, a, s);
     return hipGetLastError();
 }
 

@@ -34,7 +34,7 @@ module lbm_capi
     end type lbm_params
 
     type, bind(C) :: lbm_launch
-        integer(c_int32_t) :: bx, by, remap, occupancy
+        integer(c_int32_t) :: bx, by, remap, occupancy, variant
     end type lbm_launch
 
     interface

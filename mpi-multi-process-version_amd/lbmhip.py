@@ -47,7 +47,7 @@ class LbmError(RuntimeError):
 
 
 class Launch(ctypes.Structure):
-    _fields_ = [(k, ctypes.c_int32) for k in ("bx", "by", "remap", "occupancy")]
+    _fields_ = [(k, ctypes.c_int32) for k in ("bx", "by", "remap", "occupancy", "variant")]
 
 
 class Params(ctypes.Structure):
@@ -230,8 +230,8 @@ class Solver:
         _check(self._L.lbm_bytes_per_update(self.h, ctypes.byref(b)))
         return b.value
 
-    def set_launch(self, bx, by, remap=1, occupancy=0):
-        cfg = Launch(bx=bx, by=by, remap=remap, occupancy=occupancy)
+    def set_launch(self, bx, by, remap=1, occupancy=0, variant=0):
+        cfg = Launch(bx=bx, by=by, remap=remap, occupancy=occupancy, variant=variant)
         _check(self._L.lbm_set_launch(self.h, ctypes.byref(cfg)))
 
     def get_launch(self):

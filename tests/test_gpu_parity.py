@@ -251,3 +251,10 @@ def test_tile_orders_give_identical_results(hip, dim, n, bx, by):
     for other in outs[1:]:
         for x, y in zip(outs[0], other):
             assert np.array_equal(x, y)
+
+
+def test_launch_variant_other_than_zero_is_rejected(hip):
+    s = hip.Solver(2, 33, 1e4)
+    with pytest.raises(hip.LbmError):
+        s.set_launch(64, 4, variant=1)
+    assert s.get_launch()["variant"] == 0

@@ -122,6 +122,59 @@ def test_reference_tables_are_inverse_pairs(dim):
                 assert s == (1 if r == c else 0), (dim, a, r, c)
 
 
+# --------------------------------------------------------- initial state
+def _moments(pop, C):
+    """Zeroth, first and second moments of populations shaped (..., Q)."""
+    m0 = pop.sum(-1)
+    m1 = np.einsum("...a,ad->...d", pop, C.astype(float))
+    m2 = np.einsum("...a,ad,ae->...de", pop, C.astype(float), C.astype(float))
+    return m0, m1, m2
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_initial_condition_is_the_reference_state(oracle_mod, dim):
+    """initial (2M:199-240, 3M:256-301): rho = 1, u = 0, T linear in the node
+    INDEX between the heated walls (x in 2D, y in 3D), and populations whose
+    moments are those of the equilibria: sum f = rho, sum c f = rho u,
+    sum c c f = rho (I/3 + u u); sum g = T, sum c g = T u."""
+    n = 9
+    o = oracle_mod.Oracle(dim, n, 1e5, Thot=1.5, Tcold=-0.25)
+    o.init_rest()
+    C, W, WT, _ = lattice(dim)
+    rho, u, v, w, T = o.fields()
+    idx = np.arange(n) / (n - 1) * (-0.25 - 1.5) + 1.5
+    want = np.broadcast_to(idx if dim == 2 else idx[None, :, None], T.shape)
+    assert np.array_equal(T, want)
+    assert np.all(rho == 1.0) and not u.any() and not v.any() and not w.any()
+    m0, m1, m2 = _moments(o.array("f"), C)
+    np.testing.assert_allclose(m0, 1.0, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(m1, 0.0, rtol=0, atol=1e-16)
+    np.testing.assert_allclose(m2[..., :dim, :dim], np.broadcast_to(np.eye(dim) / 3, m2[..., :dim, :dim].shape), atol=1e-15)
+    g0, g1, _ = _moments(o.array("g"), C[:len(WT)])
+    np.testing.assert_allclose(g0, T, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(g1, 0.0, rtol=0, atol=1e-16)
+
+    # warm start from given fields (lbm_init_fields): same equilibria at u != 0
+    rng = np.random.default_rng(7)
+    shp = o.shape
+    uu = [0.03 * rng.standard_normal(shp) for _ in range(3)]
+    if dim == 2:
+        uu[2] = None
+    TT = 0.5 + 0.2 * rng.standard_normal(shp)
+    o.init_fields(uu[0], uu[1], uu[2], TT)
+    U = np.stack([uu[d] if uu[d] is not None else np.zeros(shp) for d in range(3)], -1)
+    m0, m1, m2 = _moments(o.array("f"), C)
+    np.testing.assert_allclose(m0, 1.0, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(m1, U[..., :C.shape[1]], rtol=0, atol=1e-15)
+    want2 = np.eye(C.shape[1]) / 3 + U[..., :C.shape[1], None] * U[..., None, :C.shape[1]]
+    np.testing.assert_allclose(m2[..., :dim, :dim], want2[..., :dim, :dim], rtol=0, atol=1e-14)
+    g0, g1, _ = _moments(o.array("g"), C[:len(WT)])
+    np.testing.assert_allclose(g0, TT, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(g1[..., :dim], (TT[..., None] * U[..., :C.shape[1]])[..., :dim],
+                               rtol=0, atol=1e-14)
+    o.close()
+
+
 # --------------------------------------------------------------- collision
 def _setup_random_state(o, dim, seed):
     rng = np.random.default_rng(seed)
