@@ -630,13 +630,25 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes) {
     return LBM_OK;
 }
 
+// Launch variant 1 (LDS-staged z-march) runs where the far-wall node never
+// sits in the first column or row of a tile, so that its one-sided stencil
+// stays inside the staged halo.
+static bool zm_launch_ok(const lbm_ctx &c, int bx, int by, int occ) {
+    return c.dim == 3 && !c.fp64 && bx == 64 && (by == 2 || by == 4) && occ == 0 &&
+           c.n % 64 != 1 && (c.n - 1) % by != 0;
+}
+
 int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
     if (!h || !l) return fail(LBM_ERR_ARG, "null argument");
     if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
         l->remap < 0 || l->remap > 2 ||
         (l->occupancy != 0 && l->occupancy != 5 && l->occupancy != 6) ||
-        l->variant != 0)
+        l->variant < 0 || l->variant > 1)
         return fail(LBM_ERR_ARG, "invalid launch configuration");
+    if (l->variant == 1 && !zm_launch_ok(*h, l->bx, l->by, l->occupancy))
+        return fail(LBM_ERR_ARG,
+                    "variant 1 needs 3D FP32 storage, bx = 64, by = 2 or 4, occupancy 0, "
+                    "n % 64 != 1 and (n - 1) % by != 0");
     h->shape.bx = l->bx;
     h->shape.by = l->by;
     h->shape.remap = l->remap;
@@ -689,14 +701,16 @@ int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms) {
     const int nocc = (h->dim == 3 && h->p.precision == LBM_PREC_FP32_SHIFTED) ? 3 : 1;
     for (const auto &cd : cand) {
         if (cd[0] > 2 * h->n && cd[0] > 64) continue;
-        const int nvar = 1;
-        for (int combo = 0; combo < 3 * nocc * nvar; combo++) {
+        // variant 1 (occupancy fixed by its LDS) where it can run
+        const int nvar = zm_launch_ok(*h, cd[0], cd[1], 0) ? 2 : 1;
+        for (int combo = 0; combo < 3 * (nocc + nvar - 1); combo++) {
             static const int occs[3] = {0, 5, 6};
+            const int sel = combo / 3;   // 0..nocc-1: variant 0 occupancies; nocc: variant 1
             h->shape.bx = cd[0];
             h->shape.by = cd[1];
             h->shape.remap = combo % 3;
-            h->shape.occ = occs[(combo / 3) % nocc];
-            h->shape.variant = combo / (3 * nocc);
+            h->shape.occ = sel < nocc ? occs[sel] : 0;
+            h->shape.variant = sel < nocc ? 0 : 1;
             double ms = 0;
             // one untimed warm-up launch per candidate, then the timed trial
             if ((rc = do_steps(*h, 1)) || (rc = lbm_time_steps(h, steps_per_trial, &ms))) {

@@ -18,7 +18,7 @@ struct LaunchShape {
     int bx = 64, by = 4;
     int remap = 1;
     int occ = 0;           // 3D FP32: target waves per SIMD (0, 5 or 6)
-    int variant = 0;       // FP32: 1 = x neighbours by lane shifts (bx % 64 == 0)
+    int variant = 0;       // 3D FP32: 1 = LDS-staged z-march (lbm_step_zm.h)
 };
 
 // Device-side state of one cavity.  Populations live in two ping-pong

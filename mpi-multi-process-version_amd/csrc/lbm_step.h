@@ -501,62 +501,49 @@ __device__ __forceinline__ void tile_coords(int P, int G, int gx, int gy, int re
     tz = r / gy;
 }
 
-// WPE > 0 asks the compiler for at least WPE resident waves per SIMD (it
-// then caps the registers per lane), trading load hoisting for latency
-// hiding; the autotuner picks.
-template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, int WPE = 0>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
-void k_step(StepArgs<R, S> a) {
+// Steps 2-4 of a node once f(k), g(k) are interpolated: the thermal shift
+// correction, the wall rules (with the node's own P(k) in pcf, pcg), the
+// macroscopic fields with the lagged force, the collision and the stores.
+// sp_f, sp_g are the sums of the node's own P(k) (the zeroth moments that
+// carry rho(k-1), T(k-1) on the FP32 path).
+// The shifted path's per-node constants along the heated axis: Tref at the
+// node and the streaming corrections for c = +1 and c = -1.
+template <typename R> struct HeatShift {
+    R tloc, tc_plus, tc_minus;
+};
+template <int DIM, bool SHIFT, typename R, typename S>
+__device__ __forceinline__ HeatShift<R> load_heat_shift(const StepArgs<R, S> &a,
+                                                        const int (&idx)[3]) {
+    HeatShift<R> h{R(0), R(0), R(0)};
+    if constexpr (SHIFT) {
+        const int t = idx[heated_axis<DIM>()];
+        h.tloc = a.tref[t];
+        h.tc_plus = a.tcorr[2 * t];
+        h.tc_minus = a.tcorr[2 * t + 1];
+    }
+    return h;
+}
+
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO>
+__device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node,
+                                          const int (&idx)[3], const R *pcf, const R *pcg,
+                                          R *f, R *g, R sp_f, R sp_g, const HeatShift<R> &hs) {
     constexpr bool EXACT = std::is_same<S, double>::value;
     using FL = Flow<DIM>;
     using HT = Heat<DIM>;
     constexpr int QF = FL::Q, QG = HT::Q;
     const int n = a.n;
-    int tx, ty, tz;
-    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tz);
-    const int i = tx * blockDim.x + threadIdx.x;
-    const int j = ty * blockDim.y + threadIdx.y;
-    const int k = DIM == 3 ? tz : 0;
-    if (i >= n || j >= n) return;
-    const uint32_t sy = n, sz = (uint32_t)n * n;
-    const uint32_t node = i + sy * j + sz * k;
-    const int idx[3] = {i, j, k};
-    const uint32_t stride[3] = {1, sy, sz};
-    AxisStencil<R> st[3];
-    st[0] = load_stencil(a, i);
-    st[1] = load_stencil(a, j);
-    if constexpr (DIM == 3) st[2] = load_stencil(a, k);
-
-    R pcf[QF], pcg[QG], f[QF], g[QG];
-    static_for<0, QF>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        pcf[al] = R(ldp(a.pin + al * a.plane, node));
-    });
-    static_for<0, QG>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        pcg[al] = R(ldp(a.pin + (QF + al) * a.plane, node));
-    });
-    R sp_f = R(0), sp_g = R(0);
-    static_for<0, QF>([&](auto A) { sp_f += pcf[decltype(A)::value]; });
-    static_for<0, QG>([&](auto A) { sp_g += pcg[decltype(A)::value]; });
     constexpr int HAX = heated_axis<DIM>();
-    const R tloc = SHIFT ? a.tref[idx[HAX]] : R(0);
-    f[0] = pcf[0];
-    g[0] = pcg[0];
-    static_for<1, QF>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        f[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2], EXACT>(
-            a.pin + al * a.plane, node, pcf[al], st, stride);
-    });
-    static_for<1, QG>([&](auto A) {
-        constexpr int al = decltype(A)::value;
-        g[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2], EXACT>(
-            a.pin + (QF + al) * a.plane, node, pcg[al], st, stride);
-        if constexpr (SHIFT && FL::c[al][HAX] != 0) {
-            constexpr double wa = HT::w[al];
-            g[al] += R(wa) * a.tcorr[2 * idx[HAX] + (FL::c[al][HAX] > 0 ? 0 : 1)];
-        }
-    });
+    const R tloc = hs.tloc;
+    if constexpr (SHIFT) {
+        static_for<1, QG>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            if constexpr (FL::c[al][HAX] != 0) {
+                constexpr double wa = HT::w[al];
+                g[al] += R(wa) * (FL::c[al][HAX] > 0 ? hs.tc_plus : hs.tc_minus);
+            }
+        });
+    }
     static_for<0, DIM>([&](auto AX) {
         constexpr int ax = decltype(AX)::value;
         const int side = idx[ax] == 0 ? 1 : (idx[ax] == n - 1 ? -1 : 0);
@@ -684,6 +671,60 @@ void k_step(StepArgs<R, S> a) {
         constexpr int al = decltype(A)::value;
         stp(a.pout + (QF + al) * a.plane, node, S(gp[al]));
     });
+}
+
+// WPE > 0 asks the compiler for at least WPE resident waves per SIMD (it
+// then caps the registers per lane), trading load hoisting for latency
+// hiding; the autotuner picks.
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, int WPE = 0>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_step(StepArgs<R, S> a) {
+    constexpr bool EXACT = std::is_same<S, double>::value;
+    using FL = Flow<DIM>;
+    using HT = Heat<DIM>;
+    constexpr int QF = FL::Q, QG = HT::Q;
+    const int n = a.n;
+    int tx, ty, tz;
+    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tz);
+    const int i = tx * blockDim.x + threadIdx.x;
+    const int j = ty * blockDim.y + threadIdx.y;
+    const int k = DIM == 3 ? tz : 0;
+    if (i >= n || j >= n) return;
+    const uint32_t sy = n, sz = (uint32_t)n * n;
+    const uint32_t node = i + sy * j + sz * k;
+    const int idx[3] = {i, j, k};
+    const uint32_t stride[3] = {1, sy, sz};
+    AxisStencil<R> st[3];
+    st[0] = load_stencil(a, i);
+    st[1] = load_stencil(a, j);
+    if constexpr (DIM == 3) st[2] = load_stencil(a, k);
+
+    R pcf[QF], pcg[QG], f[QF], g[QG];
+    static_for<0, QF>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        pcf[al] = R(ldp(a.pin + al * a.plane, node));
+    });
+    static_for<0, QG>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        pcg[al] = R(ldp(a.pin + (QF + al) * a.plane, node));
+    });
+    R sp_f = R(0), sp_g = R(0);
+    static_for<0, QF>([&](auto A) { sp_f += pcf[decltype(A)::value]; });
+    static_for<0, QG>([&](auto A) { sp_g += pcg[decltype(A)::value]; });
+    f[0] = pcf[0];
+    g[0] = pcg[0];
+    static_for<1, QF>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        f[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2], EXACT>(
+            a.pin + al * a.plane, node, pcf[al], st, stride);
+    });
+    static_for<1, QG>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        g[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2], EXACT>(
+            a.pin + (QF + al) * a.plane, node, pcg[al], st, stride);
+    });
+    node_tail<DIM, S, R, SHIFT, MACRO>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
+                                       load_heat_shift<DIM, SHIFT>(a, idx));
 }
 
 // Initial collision (iteration 1's collision_U/collision_T): equilibrium

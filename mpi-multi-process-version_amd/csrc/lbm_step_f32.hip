@@ -1,6 +1,9 @@
 // lbm_step_f32.hip -- FP32 shifted-population instantiation of the fused
 // step (the production path).  Built with default FMA contraction.
 #include "lbm_internal.h"
+#include "lbm_step_zm.h"
+
+#include <algorithm>
 
 namespace lbm {
 
@@ -13,8 +16,15 @@ static void launch_k(bool macro, dim3 grid, dim3 block, const StepArgs<float> &a
         hipLaunchKernelGGL((k_step<DIM, float, float, true, false, WPE>), grid, block, 0, s, a);
 }
 
+hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
+                          hipStream_t s);
+
 hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
                            hipStream_t s) {
+    if (ls.variant == 1) {
+        if (dim != 3) return hipErrorInvalidValue;
+        return launch_step_zm(macro, a0, ls, s);
+    }
     StepArgs<float> a = a0;
     dim3 block(ls.bx, ls.by, 1);
     a.gx = (a.n + ls.bx - 1) / ls.bx;
@@ -26,6 +36,36 @@ hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a0, const
     else if (ls.occ == 5) launch_k<3, 5>(macro, grid, block, a, s);
     else if (ls.occ == 6) launch_k<3, 6>(macro, grid, block, a, s);
     else launch_k<3, 0>(macro, grid, block, a, s);
+    return hipGetLastError();
+}
+
+// Launch variant 1 (3D): LDS-staged z-march, one 64 x by column of nodes
+// per workgroup over kc planes.  kc is chosen so that the launch has about
+// 2048 workgroups (8 per CU), enough to balance the chip.
+int zm_planes_per_group(int n, int by) {
+    const long cols = (long)((n + 63) / 64) * ((n + by - 1) / by);
+    long nzc = (2048 + cols - 1) / cols;
+    nzc = std::max(1L, std::min(nzc, (long)std::max(1, n / 4)));
+    return (int)((n + nzc - 1) / nzc);
+}
+
+hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
+                          hipStream_t s) {
+    if (ls.bx != 64 || (ls.by != 2 && ls.by != 4)) return hipErrorInvalidValue;
+    StepArgs<float> a = a0;
+    const int kc = zm_planes_per_group(a.n, ls.by);
+    a.gx = (a.n + 63) / 64;
+    a.gy = (a.n + ls.by - 1) / ls.by;
+    a.ntiles = a.gx * a.gy * ((a.n + kc - 1) / kc);
+    a.remap = (ls.remap == 2 && a.gy % 8) ? 0 : ls.remap;
+    dim3 block(64, ls.by, 1), grid(a.ntiles);
+    if (ls.by == 2) {
+        if (macro) hipLaunchKernelGGL((k_step_zm<2, true>), grid, block, 0, s, a, kc);
+        else hipLaunchKernelGGL((k_step_zm<2, false>), grid, block, 0, s, a, kc);
+    } else {
+        if (macro) hipLaunchKernelGGL((k_step_zm<4, true>), grid, block, 0, s, a, kc);
+        else hipLaunchKernelGGL((k_step_zm<4, false>), grid, block, 0, s, a, kc);
+    }
     return hipGetLastError();
 }
 

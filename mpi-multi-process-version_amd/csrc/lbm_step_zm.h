@@ -1,0 +1,311 @@
+// lbm_step_zm.h -- LDS-staged, z-marching variant of the 3D FP32 step
+// (launch variant 1).
+//
+// The plain kernel (k_step) reads 156 values per node through the L1; the
+// stencil rows it shares with other workgroups (the y halo and the planes
+// k +- 1) are L1 misses, and the L1's miss handling, not HBM, bounds it.
+// Here a workgroup owns a 64 x BY column of nodes over a range of planes and
+// marches it in +z.  The post-collision populations it needs are staged in
+// LDS once per plane:
+//   * populations with c_z = 0 (except the rest ones): plane k, with a one-
+//     node halo in x if c_x != 0 and in y if c_y != 0;
+//   * populations with c_z != 0: a ring of three planes, the window
+//     [ws, ws + 2] with ws = clamp(k - 1, 0, n - 3), which holds every z
+//     stencil point of plane k, one-sided wall stencils included;
+//   * the two rest populations are read by each lane directly.
+// Each plane row is then loaded once per workgroup (about 34 loads per
+// node, against 156), the next plane's rows are fetched into registers
+// while the current plane is computed, and every stencil read is an LDS read.
+// The interpolation expressions and their operands are those of k_step; the
+// results agree with it to FP32 rounding (the compiler fuses a different
+// handful of multiply-adds in the two kernels).
+//
+// Preconditions (checked by the host): FP32 storage, 3D, bx = 64, by = BY,
+// n % 64 != 1 and (n - 1) % BY != 0 (the far-wall node then never sits in
+// the first column or row of a tile, so its one-sided stencil stays inside
+// the staged halo).
+#pragma once
+
+#include "lbm_step.h"
+
+namespace lbm {
+
+// Staging slots p = 0..25: flow populations 0..18, then thermal 0..6
+// (whose velocities are the first seven flow velocities).  The population
+// plane of slot p is plane p of the buffer.
+template <int BY> struct ZmLayout {
+    static constexpr int NP = 26;
+    static constexpr int vel(int p) { return p < 19 ? p : p - 19; }
+    static constexpr int cx(int p) { return Flow<3>::c[vel(p)][0]; }
+    static constexpr int cy(int p) { return Flow<3>::c[vel(p)][1]; }
+    static constexpr int cz(int p) { return Flow<3>::c[vel(p)][2]; }
+    static constexpr bool staged(int p) { return p != 0 && p != 19; }
+    static constexpr bool zpop(int p) { return cz(p) != 0; }
+    static constexpr int hx(int p) { return cx(p) != 0 ? 1 : 0; }
+    static constexpr int hy(int p) { return cy(p) != 0 ? 1 : 0; }
+    static constexpr int W(int p) { return 64 + 2 * hx(p); }
+    static constexpr int H(int p) { return BY + 2 * hy(p); }
+    static constexpr int slots(int p) { return zpop(p) ? 3 : 1; }
+    static constexpr int psize(int p) { return staged(p) ? W(p) * H(p) : 0; }
+    static constexpr int off(int p) {
+        int o = 0;
+        for (int q = 0; q < p; q++) o += psize(q) * slots(q);
+        return o;
+    }
+    static constexpr int total() { return off(NP); }
+    // A wave stages rows w and w + BY of every population of a phase
+    // (H <= 2 BY); the second row is clamped to H - 1, so some waves load a
+    // row twice and write the same values.
+    static constexpr int npops(bool z) {
+        int c = 0;
+        for (int p = 0; p < NP; p++)
+            if (staged(p) && zpop(p) == z) c++;
+        return c;
+    }
+    static constexpr int pop_at(bool z, int q) {
+        for (int p = 0; p < NP; p++)
+            if (staged(p) && zpop(p) == z) {
+                if (q == 0) return p;
+                q--;
+            }
+        return -1;
+    }
+    static constexpr int nhalo(bool z) {
+        int h = 0;
+        for (int p = 0; p < NP; p++)
+            if (staged(p) && zpop(p) == z) h += 2 * hx(p) * H(p);
+        return h;
+    }
+};
+
+// Registers holding one phase's rows while they are in flight: for each
+// population of the phase, rows w and min(w + BY, H - 1) of its region
+// (lane l holds column l), plus one halo value per thread t < nhalo.
+template <int BY, bool Z> struct ZmStage {
+    using L = ZmLayout<BY>;
+    float row[2 * L::npops(Z)];
+    float halo;
+};
+
+// Per-thread halo assignment of a phase, fixed for the whole march: thread t
+// owns halo value t (population, region row, side), or none.
+struct ZmHalo {
+    size_t goff;     // element offset from the buffer start, without the plane
+    int lds;         // LDS index in ring slot 0, -1 if the thread owns none
+    int psz;         // region size of its population (ring stride)
+};
+
+template <int BY, bool Z>
+__device__ __forceinline__ ZmHalo zm_halo_plan(const StepArgs<float> &a, int i0, int j0) {
+    using L = ZmLayout<BY>;
+    const int n = a.n;
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    ZmHalo z{0, -1, 0};
+    int h = t;
+    static_for<0, L::NP>([&](auto P) {
+        constexpr int p = decltype(P)::value;
+        if constexpr (L::staged(p) && L::zpop(p) == Z && L::hx(p) == 1) {
+            constexpr int cnt = 2 * L::H(p);
+            const bool mine = h >= 0 && h < cnt;
+            const int rr = h >> 1, side = h & 1;
+            const int jj = j0 - L::hy(p) + rr, ii = side ? i0 + 64 : i0 - 1;
+            const bool inside = jj >= 0 && jj < n && ii >= 0 && ii < n;
+            if (mine && inside) {
+                z.goff = (size_t)p * (size_t)a.plane + (size_t)jj * n + ii;
+                z.lds = L::off(p) + rr * L::W(p) + (side ? 65 : 0);
+                z.psz = L::psize(p);
+            }
+            h -= cnt;
+        }
+    });
+    return z;
+}
+
+// Loads the rows of phase Z at plane kk (clamped to the grid) into
+// registers.  No branches: every load is issued, so the waits before the
+// compute count them as in flight.
+template <int BY, bool Z>
+__device__ __forceinline__ void zm_fetch(const StepArgs<float> &a, int kk, int i0, int j0,
+                                         const ZmHalo &hp, ZmStage<BY, Z> &s) {
+    using L = ZmLayout<BY>;
+    const int n = a.n;
+    const int l = threadIdx.x, w = threadIdx.y;
+    kk = min(kk, n - 1);
+    const uint32_t pl = (uint32_t)kk * (uint32_t)n * (uint32_t)n;
+    const int ii = min(i0 + l, n - 1);
+    static_for<0, L::npops(Z)>([&](auto Q) {
+        constexpr int q = decltype(Q)::value;
+        constexpr int p = L::pop_at(Z, q);
+        static_for<0, 2>([&](auto S) {
+            constexpr int sr = decltype(S)::value;
+            const int rr = min(w + sr * BY, L::H(p) - 1);
+            const int jj = min(max(j0 - L::hy(p) + rr, 0), n - 1);
+            s.row[2 * q + sr] = ldp(a.pin + p * a.plane, pl + (uint32_t)jj * n + ii);
+        });
+    });
+    // threads without a halo value load their own plane origin (unused)
+    s.halo = a.pin[(size_t)pl + hp.goff];
+}
+
+// Writes fetched rows into their LDS slot (Z: ring slot of the plane).
+template <int BY, bool Z>
+__device__ __forceinline__ void zm_commit(float *lds, const ZmHalo &hp, const ZmStage<BY, Z> &s,
+                                          int slot) {
+    using L = ZmLayout<BY>;
+    const int l = threadIdx.x, w = threadIdx.y;
+    static_for<0, L::npops(Z)>([&](auto Q) {
+        constexpr int q = decltype(Q)::value;
+        constexpr int p = L::pop_at(Z, q);
+        static_for<0, 2>([&](auto S) {
+            constexpr int sr = decltype(S)::value;
+            const int rr = min(w + sr * BY, L::H(p) - 1);
+            lds[L::off(p) + L::psize(p) * (Z ? slot : 0) + rr * L::W(p) + L::hx(p) + l] =
+                s.row[2 * q + sr];
+        });
+    });
+    if (hp.lds >= 0) lds[hp.lds + hp.psz * (Z ? slot : 0)] = s.halo;
+}
+
+// Value of slot p at x, y offsets (dx, dy) from the lane's node, in ring
+// slot zs (0 for populations staged on one plane).
+template <int BY, int p>
+__device__ __forceinline__ float zm_at(const float *lds, int l, int r, int dx, int dy,
+                                       int zs) {
+    using L = ZmLayout<BY>;
+    return lds[L::off(p) + L::psize(p) * zs + (r + L::hy(p) + dy) * L::W(p) + L::hx(p) + l + dx];
+}
+
+// interp_pop with its operands from LDS: the same expressions in the same
+// order.  zs0, zs1, zs2 are the ring slots of the planes k + o0, k, k + o2.
+template <int BY, int p>
+__device__ __forceinline__ float zm_interp(const float *lds, int l, int r, float fc,
+                                           const AxisStencil<float> *st, const int *zsl) {
+    using L = ZmLayout<BY>;
+    constexpr int c[3] = {L::cx(p), L::cy(p), L::cz(p)};
+    constexpr int nz = (c[0] != 0) + (c[1] != 0) + (c[2] != 0);
+    // offset triple (dx, dy, zslot) of stencil point s (0, 1, 2) along axis ax
+    auto val = [&](int ax1, int s1, int ax2, int s2) {
+        int d[3] = {0, 0, 0};
+        int zs = zsl[1];
+        auto put = [&](int ax, int s) {
+            if (ax < 0 || s == 1) return;
+            const int o = s == 0 ? st[ax].o0 : st[ax].o2;
+            if (ax == 2) zs = zsl[s];
+            else d[ax] = o;
+        };
+        put(ax1, s1);
+        put(ax2, s2);
+        return zm_at<BY, p>(lds, l, r, d[0], d[1], L::zpop(p) ? zs : 0);
+    };
+    if constexpr (nz == 1) {
+        constexpr int ax = c[0] ? 0 : (c[1] ? 1 : 2);
+        const float *w = c[ax] > 0 ? st[ax].wp : st[ax].wm;
+        return lerp3<false>(w, val(ax, 0, -1, 1), fc, val(ax, 2, -1, 1));
+    } else {
+        constexpr int q = c[0] ? 0 : 1;  // outer stage axis
+        constexpr int pp = c[2] ? 2 : 1; // inner stage axis
+        const float *wpp = c[pp] > 0 ? st[pp].wp : st[pp].wm;
+        const float *wqq = c[q] > 0 ? st[q].wp : st[q].wm;
+        const float c0 = lerp3<false>(wpp, val(q, 0, pp, 0), val(q, 0, pp, 1), val(q, 0, pp, 2));
+        const float c1 = lerp3<false>(wpp, val(q, 1, pp, 0), fc, val(q, 1, pp, 2));
+        const float c2 = lerp3<false>(wpp, val(q, 2, pp, 0), val(q, 2, pp, 1), val(q, 2, pp, 2));
+        return lerp3<false>(wqq, c0, c1, c2);
+    }
+}
+
+__device__ __forceinline__ int zm_window(int k, int n) { return min(max(k - 1, 0), n - 3); }
+
+template <int BY, bool MACRO>
+__global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) {
+    using L = ZmLayout<BY>;
+    using FL = Flow<3>;
+    using HT = Heat<3>;
+    constexpr int QF = FL::Q, QG = HT::Q;
+    __shared__ float lds[L::total()];
+    const int n = a.n;
+    int tx, ty, tc;
+    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tc);
+    const int l = threadIdx.x, r = threadIdx.y;
+    const int i0 = tx * 64, j0 = ty * BY;
+    const int k0 = tc * kc, k1 = min(n, k0 + kc);
+    const bool live = i0 + l < n && j0 + r < n;
+    // lanes past the grid edge stage and compute like the last node and
+    // store nothing (every lane must reach the barriers)
+    const int i = min(i0 + l, n - 1), j = min(j0 + r, n - 1);
+    AxisStencil<float> st[3];
+    st[0] = load_stencil(a, i);
+    st[1] = load_stencil(a, j);
+    const ZmHalo hin = zm_halo_plan<BY, false>(a, i0, j0);
+    const ZmHalo hz = zm_halo_plan<BY, true>(a, i0, j0);
+
+    {   // prologue: the z window of plane k0 and the in-plane rows of k0
+        const int ws = zm_window(k0, n);
+        for (int zz = ws; zz < ws + 3; zz++) {
+            ZmStage<BY, true> sz;
+            zm_fetch<BY, true>(a, zz, i0, j0, hz, sz);
+            zm_commit<BY, true>(lds, hz, sz, zz % 3);
+        }
+        ZmStage<BY, false> sp;
+        zm_fetch<BY, false>(a, k0, i0, j0, hin, sp);
+        zm_commit<BY, false>(lds, hin, sp, 0);
+    }
+    __syncthreads();
+
+    const uint32_t sy = n, sz2 = (uint32_t)n * n;
+    // constant along z (the heated axis is y)
+    const HeatShift<float> hs = load_heat_shift<3, true>(a, {i, j, 0});
+    for (int k = k0; k < k1; k++) {
+        const int kn = k + 1;
+        const bool more = kn < k1;
+        const int wsn = zm_window(kn, n);
+        const bool newz = more && wsn != zm_window(k, n);
+        st[2] = load_stencil(a, k);
+        const int zsl[3] = {(k + st[2].o0) % 3, k % 3, (k + st[2].o2) % 3};
+        const uint32_t node = i + sy * j + sz2 * k;
+        const int idx[3] = {i, j, k};
+        float pcf[QF], pcg[QG], f[QF], g[QG];
+        // This plane's own loads go out before the next plane's rows: vector
+        // loads complete in order, so the compute below waits only for them
+        // and the prefetch stays in flight behind it.  The prefetch is issued
+        // unconditionally (clamped planes) for the same reason.
+        pcf[0] = ldp(a.pin, node);
+        pcg[0] = ldp(a.pin + QF * a.plane, node);
+        __builtin_amdgcn_sched_barrier(0);
+        ZmStage<BY, false> np;
+        ZmStage<BY, true> nzs;
+        zm_fetch<BY, false>(a, kn, i0, j0, hin, np);
+        zm_fetch<BY, true>(a, wsn + 2, i0, j0, hz, nzs);
+        __builtin_amdgcn_sched_barrier(0);
+
+        static_for<1, L::NP>([&](auto P) {
+            constexpr int p = decltype(P)::value;
+            if constexpr (L::staged(p)) {
+                const float v = zm_at<BY, p>(lds, l, r, 0, 0, L::zpop(p) ? k % 3 : 0);
+                if constexpr (p < QF) pcf[p] = v;
+                else pcg[p - QF] = v;
+            }
+        });
+        float sp_f = 0.0f, sp_g = 0.0f;
+        static_for<0, QF>([&](auto A) { sp_f += pcf[decltype(A)::value]; });
+        static_for<0, QG>([&](auto A) { sp_g += pcg[decltype(A)::value]; });
+        f[0] = pcf[0];
+        g[0] = pcg[0];
+        static_for<1, QF>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            f[al] = zm_interp<BY, al>(lds, l, r, pcf[al], st, zsl);
+        });
+        static_for<1, QG>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            g[al] = zm_interp<BY, QF + al>(lds, l, r, pcg[al], st, zsl);
+        });
+        if (live)
+            node_tail<3, float, float, true, MACRO>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g, hs);
+
+        __syncthreads();   // every read of plane k's staging is done
+        if (more) zm_commit<BY, false>(lds, hin, np, 0);
+        if (newz) zm_commit<BY, true>(lds, hz, nzs, (wsn + 2) % 3);
+        __syncthreads();
+    }
+}
+
+}  // namespace lbm

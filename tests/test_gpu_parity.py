@@ -253,8 +253,73 @@ def test_tile_orders_give_identical_results(hip, dim, n, bx, by):
             assert np.array_equal(x, y)
 
 
-def test_launch_variant_other_than_zero_is_rejected(hip):
-    s = hip.Solver(2, 33, 1e4)
-    with pytest.raises(hip.LbmError):
-        s.set_launch(64, 4, variant=1)
-    assert s.get_launch()["variant"] == 0
+
+@pytest.mark.parametrize("n,by,remap", [
+    (12, 2, 0),     # one plane chunk per 4 planes, walls in every tile
+    (18, 4, 0),     # far wall at tile row 1
+    (66, 4, 1),     # a partial second column of lanes, far wall at lane 1
+    (70, 2, 2),
+    (96, 4, 0),
+    (130, 4, 2),    # y bands per XCD
+])
+def test_lds_zmarch_variant_gives_identical_results(hip, n, by, remap):
+    """Variant 1 stages the populations in LDS and marches in z; its
+    interpolation reads the same values in the same order as variant 0.  The
+    compiler fuses a different handful of multiply-adds in the two kernels,
+    so they agree to FP32 rounding, not to the bit: ~1e-7 of the velocity
+    scale after 24 steps, while a wrong stencil point or plane shows as an
+    error near 1e-2.  Bound: 1e-6 relative to each field's largest value,
+    walls and partial tiles included; accuracy against the oracle is checked
+    separately."""
+    print(f"\n[z-march] n={n} by={by} remap={remap}", flush=True)
+    outs = []
+    for variant, bx in ((0, 64), (1, 64)):
+        s = hip.Solver(3, n, 1e5)
+        s.set_launch(bx, by, remap=remap, variant=variant)
+        assert s.get_launch()["variant"] == variant
+        s.init_rest()
+        s.step(24)
+        outs.append(s.fields())
+        s.close()
+    vscale = max(np.abs(outs[0][k]).max() for k in (1, 2, 3))
+    for name, x, y, scale in zip(("rho", "u", "v", "w", "T"), outs[0], outs[1],
+                                 (1.0, vscale, vscale, vscale, np.abs(outs[0][4]).max())):
+        err = np.abs(x - y).max() / scale
+        print(f"  {name}: {err:.3g}", flush=True)
+        assert err <= 1e-6, (name, err)
+
+
+@pytest.mark.parametrize("n,steps", [(18, 300), (34, 200)])
+def test_lds_zmarch_variant_matches_oracle(oracle_mod, hip, n, steps):
+    """Variant 1 against the FP64 oracle, to the FP32 bar of the plain path."""
+    print(f"\n[z-march vs oracle] n={n} steps={steps}", flush=True)
+    s = hip.Solver(3, n, 1e5)
+    s.set_launch(64, 2, remap=0, variant=1)
+    o = oracle_mod.Oracle(3, n, 1e5)
+    s.init_rest()
+    o.init_rest()
+    s.step(steps)
+    o.step(steps)
+    errs = field_errors(s.fields(), o.fields())
+    print(f"  {errs}", flush=True)
+    for k in ("rho", "u", "v", "w", "T"):
+        assert errs[k] <= 1e-5, (k, errs[k])
+    nh, nc = s.nusselt()
+    oh, oc = o.nusselt()
+    assert abs(nh - oh) <= 1e-4 * abs(oh) and abs(nc - oc) <= 1e-4 * abs(oc)
+    s.close()
+    o.close()
+
+
+def test_lds_zmarch_variant_is_rejected_where_it_cannot_run(hip):
+    cases = [(dict(dim=2, n=64), (64, 4)),            # 3D only
+             (dict(dim=3, n=64, precision=hip.PREC_FP64), (64, 4)),
+             (dict(dim=3, n=64), (128, 2)),          # bx must be 64
+             (dict(dim=3, n=64), (64, 8)),           # by 2 or 4
+             (dict(dim=3, n=65), (64, 4)),           # far wall at lane 0
+             (dict(dim=3, n=17), (64, 4))]           # far wall at tile row 0
+    for kw, (bx, by) in cases:
+        s = hip.Solver(kw["dim"], kw["n"], 1e5, precision=kw.get("precision", hip.PREC_FP32_SHIFTED))
+        with pytest.raises(hip.LbmError):
+            s.set_launch(bx, by, variant=1)
+        s.close()
