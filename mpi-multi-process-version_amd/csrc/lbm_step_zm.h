@@ -13,15 +13,16 @@
 //     [ws, ws + 2] with ws = clamp(k - 1, 0, n - 3), which holds every z
 //     stencil point of plane k, one-sided wall stencils included;
 //   * the two rest populations are read by each lane directly.
-// Each plane row is then loaded once per workgroup (about 34 loads per
-// node, against 156), the next plane's rows are fetched into registers
-// while the current plane is computed, and every stencil read is an LDS read.
-// The interpolation expressions and their operands are those of k_step; the
-// results agree with it to FP32 rounding (the compiler fuses a different
-// handful of multiply-adds in the two kernels).
+// Rows are staged four at a time: a lane loads four consecutive x values
+// (one 16-byte load), so a wave moves four 64-wide rows per instruction, and
+// the rows of the next plane are fetched into registers while the current
+// plane is computed.  Every stencil read is an LDS read.  The interpolation
+// expressions and their operands are those of k_step; the results agree
+// with it to FP32 rounding (the compiler fuses a different handful of
+// multiply-adds in the two kernels).
 //
 // Preconditions (checked by the host): FP32 storage, 3D, bx = 64, by = BY,
-// n % 64 != 1 and (n - 1) % BY != 0 (the far-wall node then never sits in
+// n % 4 == 0 (16-byte aligned rows; the far-wall node then never sits in
 // the first column or row of a tile, so its one-sided stencil stays inside
 // the staged halo).
 #pragma once
@@ -30,9 +31,17 @@
 
 namespace lbm {
 
+// A native 4-float vector (HIP's float4 is a union type that the compiler
+// does not keep in registers when it sits in an array).
+typedef float zm_f4 __attribute__((ext_vector_type(4)));
+typedef float zm_f2 __attribute__((ext_vector_type(2)));
+
 // Staging slots p = 0..25: flow populations 0..18, then thermal 0..6
 // (whose velocities are the first seven flow velocities).  The population
-// plane of slot p is plane p of the buffer.
+// plane of slot p is plane p of the buffer.  A staged region is H rows of
+// pitch P: x columns i0 .. i0+63 at C0 .. C0+63, and for populations with
+// an x halo, column i0-1 at 3 and column i0+64 at 68 (so rows stay 16-byte
+// aligned).
 template <int BY> struct ZmLayout {
     static constexpr int NP = 26;
     static constexpr int vel(int p) { return p < 19 ? p : p - 19; }
@@ -43,47 +52,55 @@ template <int BY> struct ZmLayout {
     static constexpr bool zpop(int p) { return cz(p) != 0; }
     static constexpr int hx(int p) { return cx(p) != 0 ? 1 : 0; }
     static constexpr int hy(int p) { return cy(p) != 0 ? 1 : 0; }
-    static constexpr int W(int p) { return 64 + 2 * hx(p); }
+    static constexpr int P(int p) { return hx(p) ? 72 : 64; }
+    static constexpr int C0(int p) { return hx(p) ? 4 : 0; }
     static constexpr int H(int p) { return BY + 2 * hy(p); }
     static constexpr int slots(int p) { return zpop(p) ? 3 : 1; }
-    static constexpr int psize(int p) { return staged(p) ? W(p) * H(p) : 0; }
+    static constexpr int psize(int p) { return staged(p) ? P(p) * H(p) : 0; }
     static constexpr int off(int p) {
         int o = 0;
         for (int q = 0; q < p; q++) o += psize(q) * slots(q);
         return o;
     }
     static constexpr int total() { return off(NP); }
-    // A wave stages rows w and w + BY of every population of a phase
-    // (H <= 2 BY); the second row is clamped to H - 1, so some waves load a
-    // row twice and write the same values.
-    static constexpr int npops(bool z) {
+    // Row quads of a phase (Z: the c_z != 0 populations at a new plane;
+    // !Z: the in-plane populations at plane k): quad m covers rows
+    // 4 qd .. 4 qd + 3 of one population (rows past H - 1 repeat row H - 1).
+    static constexpr int nquads(bool z) {
         int c = 0;
         for (int p = 0; p < NP; p++)
-            if (staged(p) && zpop(p) == z) c++;
+            if (staged(p) && zpop(p) == z) c += (H(p) + 3) / 4;
         return c;
     }
-    static constexpr int pop_at(bool z, int q) {
+    static constexpr int quad_pop(bool z, int m) {
         for (int p = 0; p < NP; p++)
             if (staged(p) && zpop(p) == z) {
-                if (q == 0) return p;
-                q--;
+                const int nq = (H(p) + 3) / 4;
+                if (m < nq) return p;
+                m -= nq;
             }
         return -1;
     }
-    static constexpr int nhalo(bool z) {
-        int h = 0;
+    static constexpr int quad_row0(bool z, int m) {
         for (int p = 0; p < NP; p++)
-            if (staged(p) && zpop(p) == z) h += 2 * hx(p) * H(p);
-        return h;
+            if (staged(p) && zpop(p) == z) {
+                const int nq = (H(p) + 3) / 4;
+                if (m < nq) return 4 * m;
+                m -= nq;
+            }
+        return -1;
+    }
+    // quads of wave W: m = W, W + BY, ...
+    static constexpr int wave_quads(bool z, int W) {
+        return W < nquads(z) ? (nquads(z) - W + BY - 1) / BY : 0;
     }
 };
 
-// Registers holding one phase's rows while they are in flight: for each
-// population of the phase, rows w and min(w + BY, H - 1) of its region
-// (lane l holds column l), plus one halo value per thread t < nhalo.
-template <int BY, bool Z> struct ZmStage {
+// Registers holding one wave's quads of one phase while they are in flight,
+// plus the thread's halo value.
+template <int BY, bool Z, int W> struct ZmStage {
     using L = ZmLayout<BY>;
-    float row[2 * L::npops(Z)];
+    zm_f4 quad[L::wave_quads(Z, W) > 0 ? L::wave_quads(Z, W) : 1];
     float halo;
 };
 
@@ -96,14 +113,14 @@ struct ZmHalo {
 };
 
 template <int BY, bool Z>
-__device__ __forceinline__ ZmHalo zm_halo_plan(const StepArgs<float> &a, int i0, int j0) {
+__device__ __forceinline__ ZmHalo zm_halo_plan(const StepArgs<float> &a, int i0, int j0,
+                                               int t) {
     using L = ZmLayout<BY>;
     const int n = a.n;
-    const int t = threadIdx.x + 64 * threadIdx.y;
     ZmHalo z{0, -1, 0};
     int h = t;
-    static_for<0, L::NP>([&](auto P) {
-        constexpr int p = decltype(P)::value;
+    static_for<0, L::NP>([&](auto Pp) {
+        constexpr int p = decltype(Pp)::value;
         if constexpr (L::staged(p) && L::zpop(p) == Z && L::hx(p) == 1) {
             constexpr int cnt = 2 * L::H(p);
             const bool mine = h >= 0 && h < cnt;
@@ -112,7 +129,7 @@ __device__ __forceinline__ ZmHalo zm_halo_plan(const StepArgs<float> &a, int i0,
             const bool inside = jj >= 0 && jj < n && ii >= 0 && ii < n;
             if (mine && inside) {
                 z.goff = (size_t)p * (size_t)a.plane + (size_t)jj * n + ii;
-                z.lds = L::off(p) + rr * L::W(p) + (side ? 65 : 0);
+                z.lds = L::off(p) + rr * L::P(p) + (side ? 68 : 3);
                 z.psz = L::psize(p);
             }
             h -= cnt;
@@ -121,47 +138,47 @@ __device__ __forceinline__ ZmHalo zm_halo_plan(const StepArgs<float> &a, int i0,
     return z;
 }
 
-// Loads the rows of phase Z at plane kk (clamped to the grid) into
-// registers.  No branches: every load is issued, so the waits before the
-// compute count them as in flight.
-template <int BY, bool Z>
+// Loads wave W's quads of phase Z at plane kk (clamped to the grid) into
+// registers.  Straight-line code: every load is issued, so the waits before
+// the compute count them as in flight.
+template <int BY, bool Z, int W>
 __device__ __forceinline__ void zm_fetch(const StepArgs<float> &a, int kk, int i0, int j0,
-                                         const ZmHalo &hp, ZmStage<BY, Z> &s) {
+                                         const ZmHalo &hp, ZmStage<BY, Z, W> &s) {
     using L = ZmLayout<BY>;
     const int n = a.n;
-    const int l = threadIdx.x, w = threadIdx.y;
+    const int l = threadIdx.x, rsub = l >> 4, c4 = 4 * (l & 15);
     kk = min(kk, n - 1);
     const uint32_t pl = (uint32_t)kk * (uint32_t)n * (uint32_t)n;
-    const int ii = min(i0 + l, n - 1);
-    static_for<0, L::npops(Z)>([&](auto Q) {
+    const int ii = min(i0 + c4, n - 4);   // n % 4 == 0: groups never straddle the edge
+    static_for<0, L::wave_quads(Z, W)>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
-        constexpr int p = L::pop_at(Z, q);
-        static_for<0, 2>([&](auto S) {
-            constexpr int sr = decltype(S)::value;
-            const int rr = min(w + sr * BY, L::H(p) - 1);
-            const int jj = min(max(j0 - L::hy(p) + rr, 0), n - 1);
-            s.row[2 * q + sr] = ldp(a.pin + p * a.plane, pl + (uint32_t)jj * n + ii);
-        });
+        constexpr int m = W + BY * q;
+        constexpr int p = L::quad_pop(Z, m), row0 = L::quad_row0(Z, m);
+        const int rr = min(row0 + rsub, L::H(p) - 1);
+        const int jj = min(max(j0 - L::hy(p) + rr, 0), n - 1);
+        s.quad[q] = *reinterpret_cast<const zm_f4 *>(
+            reinterpret_cast<const char *>(a.pin + p * a.plane) +
+            (pl + (uint32_t)jj * n + ii) * (uint32_t)sizeof(float));
     });
     // threads without a halo value load their own plane origin (unused)
     s.halo = a.pin[(size_t)pl + hp.goff];
 }
 
-// Writes fetched rows into their LDS slot (Z: ring slot of the plane).
-template <int BY, bool Z>
-__device__ __forceinline__ void zm_commit(float *lds, const ZmHalo &hp, const ZmStage<BY, Z> &s,
-                                          int slot) {
+// Writes fetched quads into their LDS slot (Z: ring slot of the plane).
+template <int BY, bool Z, int W>
+__device__ __forceinline__ void zm_commit(float *lds, const ZmHalo &hp,
+                                          const ZmStage<BY, Z, W> &s, int slot) {
     using L = ZmLayout<BY>;
-    const int l = threadIdx.x, w = threadIdx.y;
-    static_for<0, L::npops(Z)>([&](auto Q) {
+    const int l = threadIdx.x, rsub = l >> 4, c4 = 4 * (l & 15);
+    static_for<0, L::wave_quads(Z, W)>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
-        constexpr int p = L::pop_at(Z, q);
-        static_for<0, 2>([&](auto S) {
-            constexpr int sr = decltype(S)::value;
-            const int rr = min(w + sr * BY, L::H(p) - 1);
-            lds[L::off(p) + L::psize(p) * (Z ? slot : 0) + rr * L::W(p) + L::hx(p) + l] =
-                s.row[2 * q + sr];
-        });
+        constexpr int m = W + BY * q;
+        constexpr int p = L::quad_pop(Z, m), row0 = L::quad_row0(Z, m);
+        const int rr = min(row0 + rsub, L::H(p) - 1);
+        // two 8-byte writes: fewer LDS bank conflicts than one 16-byte write
+        float *dst = &lds[L::off(p) + L::psize(p) * (Z ? slot : 0) + rr * L::P(p) + L::C0(p) + c4];
+        *reinterpret_cast<zm_f2 *>(dst) = s.quad[q].xy;
+        *reinterpret_cast<zm_f2 *>(dst + 2) = s.quad[q].zw;
     });
     if (hp.lds >= 0) lds[hp.lds + hp.psz * (Z ? slot : 0)] = s.halo;
 }
@@ -172,62 +189,70 @@ template <int BY, int p>
 __device__ __forceinline__ float zm_at(const float *lds, int l, int r, int dx, int dy,
                                        int zs) {
     using L = ZmLayout<BY>;
-    return lds[L::off(p) + L::psize(p) * zs + (r + L::hy(p) + dy) * L::W(p) + L::hx(p) + l + dx];
+    return lds[L::off(p) + L::psize(p) * zs + (r + L::hy(p) + dy) * L::P(p) + L::C0(p) + l +
+               dx];
+}
+
+// Value of slot p at stencil point S1 along axis A1 and S2 along axis A2
+// (point 1 is the node itself; A < 0 means no axis).  zsl[s] is the ring
+// slot of the plane of z stencil point s.
+template <int BY, int p, int A1, int S1, int A2, int S2>
+__device__ __forceinline__ float zm_pt(const float *lds, int l, int r,
+                                       const AxisStencil<float> *st, const int *zsl) {
+    using L = ZmLayout<BY>;
+    int dx = 0, dy = 0, zs = zsl[1];
+    if constexpr (A1 == 0 && S1 != 1) dx = S1 == 0 ? st[0].o0 : st[0].o2;
+    if constexpr (A2 == 0 && S2 != 1) dx = S2 == 0 ? st[0].o0 : st[0].o2;
+    if constexpr (A1 == 1 && S1 != 1) dy = S1 == 0 ? st[1].o0 : st[1].o2;
+    if constexpr (A2 == 1 && S2 != 1) dy = S2 == 0 ? st[1].o0 : st[1].o2;
+    if constexpr (A1 == 2 && S1 != 1) zs = zsl[S1];
+    if constexpr (A2 == 2 && S2 != 1) zs = zsl[S2];
+    return zm_at<BY, p>(lds, l, r, dx, dy, L::zpop(p) ? zs : 0);
 }
 
 // interp_pop with its operands from LDS: the same expressions in the same
-// order.  zs0, zs1, zs2 are the ring slots of the planes k + o0, k, k + o2.
+// order.
 template <int BY, int p>
 __device__ __forceinline__ float zm_interp(const float *lds, int l, int r, float fc,
                                            const AxisStencil<float> *st, const int *zsl) {
     using L = ZmLayout<BY>;
     constexpr int c[3] = {L::cx(p), L::cy(p), L::cz(p)};
     constexpr int nz = (c[0] != 0) + (c[1] != 0) + (c[2] != 0);
-    // offset triple (dx, dy, zslot) of stencil point s (0, 1, 2) along axis ax
-    auto val = [&](int ax1, int s1, int ax2, int s2) {
-        int d[3] = {0, 0, 0};
-        int zs = zsl[1];
-        auto put = [&](int ax, int s) {
-            if (ax < 0 || s == 1) return;
-            const int o = s == 0 ? st[ax].o0 : st[ax].o2;
-            if (ax == 2) zs = zsl[s];
-            else d[ax] = o;
-        };
-        put(ax1, s1);
-        put(ax2, s2);
-        return zm_at<BY, p>(lds, l, r, d[0], d[1], L::zpop(p) ? zs : 0);
-    };
     if constexpr (nz == 1) {
         constexpr int ax = c[0] ? 0 : (c[1] ? 1 : 2);
         const float *w = c[ax] > 0 ? st[ax].wp : st[ax].wm;
-        return lerp3<false>(w, val(ax, 0, -1, 1), fc, val(ax, 2, -1, 1));
+        return lerp3<false>(w, zm_pt<BY, p, ax, 0, -1, 1>(lds, l, r, st, zsl), fc,
+                            zm_pt<BY, p, ax, 2, -1, 1>(lds, l, r, st, zsl));
     } else {
-        constexpr int q = c[0] ? 0 : 1;  // outer stage axis
-        constexpr int pp = c[2] ? 2 : 1; // inner stage axis
+        constexpr int q = c[0] ? 0 : 1;   // outer stage axis
+        constexpr int pp = c[2] ? 2 : 1;  // inner stage axis
         const float *wpp = c[pp] > 0 ? st[pp].wp : st[pp].wm;
         const float *wqq = c[q] > 0 ? st[q].wp : st[q].wm;
-        const float c0 = lerp3<false>(wpp, val(q, 0, pp, 0), val(q, 0, pp, 1), val(q, 0, pp, 2));
-        const float c1 = lerp3<false>(wpp, val(q, 1, pp, 0), fc, val(q, 1, pp, 2));
-        const float c2 = lerp3<false>(wpp, val(q, 2, pp, 0), val(q, 2, pp, 1), val(q, 2, pp, 2));
+        const float c0 = lerp3<false>(wpp, zm_pt<BY, p, q, 0, pp, 0>(lds, l, r, st, zsl),
+                                      zm_pt<BY, p, q, 0, pp, 1>(lds, l, r, st, zsl),
+                                      zm_pt<BY, p, q, 0, pp, 2>(lds, l, r, st, zsl));
+        const float c1 = lerp3<false>(wpp, zm_pt<BY, p, q, 1, pp, 0>(lds, l, r, st, zsl), fc,
+                                      zm_pt<BY, p, q, 1, pp, 2>(lds, l, r, st, zsl));
+        const float c2 = lerp3<false>(wpp, zm_pt<BY, p, q, 2, pp, 0>(lds, l, r, st, zsl),
+                                      zm_pt<BY, p, q, 2, pp, 1>(lds, l, r, st, zsl),
+                                      zm_pt<BY, p, q, 2, pp, 2>(lds, l, r, st, zsl));
         return lerp3<false>(wqq, c0, c1, c2);
     }
 }
 
 __device__ __forceinline__ int zm_window(int k, int n) { return min(max(k - 1, 0), n - 3); }
 
-template <int BY, bool MACRO>
-__global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) {
+// The march of wave W of a workgroup (the caller branches on the wave index
+// once, outside the loop, so each wave's loop is straight-line code).
+template <int BY, bool MACRO, int W>
+__device__ __forceinline__ void zm_march(const StepArgs<float> &a, float *lds, int i0, int j0,
+                                         int k0, int k1) {
     using L = ZmLayout<BY>;
     using FL = Flow<3>;
     using HT = Heat<3>;
     constexpr int QF = FL::Q, QG = HT::Q;
-    __shared__ float lds[L::total()];
     const int n = a.n;
-    int tx, ty, tc;
-    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tc);
-    const int l = threadIdx.x, r = threadIdx.y;
-    const int i0 = tx * 64, j0 = ty * BY;
-    const int k0 = tc * kc, k1 = min(n, k0 + kc);
+    const int l = threadIdx.x, r = W;
     const bool live = i0 + l < n && j0 + r < n;
     // lanes past the grid edge stage and compute like the last node and
     // store nothing (every lane must reach the barriers)
@@ -235,19 +260,19 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) 
     AxisStencil<float> st[3];
     st[0] = load_stencil(a, i);
     st[1] = load_stencil(a, j);
-    const ZmHalo hin = zm_halo_plan<BY, false>(a, i0, j0);
-    const ZmHalo hz = zm_halo_plan<BY, true>(a, i0, j0);
+    const ZmHalo hin = zm_halo_plan<BY, false>(a, i0, j0, l + 64 * W);
+    const ZmHalo hz = zm_halo_plan<BY, true>(a, i0, j0, l + 64 * W);
 
     {   // prologue: the z window of plane k0 and the in-plane rows of k0
         const int ws = zm_window(k0, n);
         for (int zz = ws; zz < ws + 3; zz++) {
-            ZmStage<BY, true> sz;
-            zm_fetch<BY, true>(a, zz, i0, j0, hz, sz);
-            zm_commit<BY, true>(lds, hz, sz, zz % 3);
+            ZmStage<BY, true, W> sz;
+            zm_fetch<BY, true, W>(a, zz, i0, j0, hz, sz);
+            zm_commit<BY, true, W>(lds, hz, sz, zz % 3);
         }
-        ZmStage<BY, false> sp;
-        zm_fetch<BY, false>(a, k0, i0, j0, hin, sp);
-        zm_commit<BY, false>(lds, hin, sp, 0);
+        ZmStage<BY, false, W> sp;
+        zm_fetch<BY, false, W>(a, k0, i0, j0, hin, sp);
+        zm_commit<BY, false, W>(lds, hin, sp, 0);
     }
     __syncthreads();
 
@@ -271,14 +296,14 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) 
         pcf[0] = ldp(a.pin, node);
         pcg[0] = ldp(a.pin + QF * a.plane, node);
         __builtin_amdgcn_sched_barrier(0);
-        ZmStage<BY, false> np;
-        ZmStage<BY, true> nzs;
-        zm_fetch<BY, false>(a, kn, i0, j0, hin, np);
-        zm_fetch<BY, true>(a, wsn + 2, i0, j0, hz, nzs);
+        ZmStage<BY, false, W> np;
+        ZmStage<BY, true, W> nzs;
+        zm_fetch<BY, false, W>(a, kn, i0, j0, hin, np);
+        zm_fetch<BY, true, W>(a, wsn + 2, i0, j0, hz, nzs);
         __builtin_amdgcn_sched_barrier(0);
 
-        static_for<1, L::NP>([&](auto P) {
-            constexpr int p = decltype(P)::value;
+        static_for<1, L::NP>([&](auto Pp) {
+            constexpr int p = decltype(Pp)::value;
             if constexpr (L::staged(p)) {
                 const float v = zm_at<BY, p>(lds, l, r, 0, 0, L::zpop(p) ? k % 3 : 0);
                 if constexpr (p < QF) pcf[p] = v;
@@ -302,10 +327,27 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) 
             node_tail<3, float, float, true, MACRO>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g, hs);
 
         __syncthreads();   // every read of plane k's staging is done
-        if (more) zm_commit<BY, false>(lds, hin, np, 0);
-        if (newz) zm_commit<BY, true>(lds, hz, nzs, (wsn + 2) % 3);
+        if (more) zm_commit<BY, false, W>(lds, hin, np, 0);
+        if (newz) zm_commit<BY, true, W>(lds, hz, nzs, (wsn + 2) % 3);
         __syncthreads();
     }
+}
+
+template <int BY, bool MACRO>
+__global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) {
+    using L = ZmLayout<BY>;
+    __shared__ zm_f4 lds4[L::total() / 4];
+    float *lds = reinterpret_cast<float *>(lds4);
+    const int n = a.n;
+    int tx, ty, tc;
+    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tc);
+    const int i0 = tx * 64, j0 = ty * BY;
+    const int k0 = tc * kc, k1 = min(n, k0 + kc);
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    static_for<0, BY>([&](auto WW) {
+        constexpr int W = decltype(WW)::value;
+        if (w == W) zm_march<BY, MACRO, W>(a, lds, i0, j0, k0, k1);
+    });
 }
 
 }  // namespace lbm

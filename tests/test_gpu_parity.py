@@ -256,11 +256,11 @@ def test_tile_orders_give_identical_results(hip, dim, n, bx, by):
 
 @pytest.mark.parametrize("n,by,remap", [
     (12, 2, 0),     # one plane chunk per 4 planes, walls in every tile
-    (18, 4, 0),     # far wall at tile row 1
-    (66, 4, 1),     # a partial second column of lanes, far wall at lane 1
-    (70, 2, 2),
+    (20, 4, 0),     # far wall at tile row 3
+    (68, 4, 1),     # a partial second column of lanes, far wall at lane 3
+    (72, 2, 2),
     (96, 4, 0),
-    (130, 4, 2),    # y bands per XCD
+    (132, 4, 2),    # y bands per XCD
 ])
 def test_lds_zmarch_variant_gives_identical_results(hip, n, by, remap):
     """Variant 1 stages the populations in LDS and marches in z; its
@@ -289,7 +289,7 @@ def test_lds_zmarch_variant_gives_identical_results(hip, n, by, remap):
         assert err <= 1e-6, (name, err)
 
 
-@pytest.mark.parametrize("n,steps", [(18, 300), (34, 200)])
+@pytest.mark.parametrize("n,steps", [(16, 300), (36, 200)])
 def test_lds_zmarch_variant_matches_oracle(oracle_mod, hip, n, steps):
     """Variant 1 against the FP64 oracle, to the FP32 bar of the plain path."""
     print(f"\n[z-march vs oracle] n={n} steps={steps}", flush=True)
@@ -316,8 +316,8 @@ def test_lds_zmarch_variant_is_rejected_where_it_cannot_run(hip):
              (dict(dim=3, n=64, precision=hip.PREC_FP64), (64, 4)),
              (dict(dim=3, n=64), (128, 2)),          # bx must be 64
              (dict(dim=3, n=64), (64, 8)),           # by 2 or 4
-             (dict(dim=3, n=65), (64, 4)),           # far wall at lane 0
-             (dict(dim=3, n=17), (64, 4))]           # far wall at tile row 0
+             (dict(dim=3, n=66), (64, 4)),           # rows not 16-byte aligned
+             (dict(dim=3, n=65), (64, 2))]
     for kw, (bx, by) in cases:
         s = hip.Solver(kw["dim"], kw["n"], 1e5, precision=kw.get("precision", hip.PREC_FP32_SHIFTED))
         with pytest.raises(hip.LbmError):
