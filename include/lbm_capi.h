@@ -181,7 +181,8 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes);
  *  variant: step-kernel variant.  0 = one node per thread, every stencil
  *    point read through the caches.  1 (3D, FP32 storage) = each workgroup
  *    marches a 64 x by column of nodes in z with the populations staged in
- *    LDS; needs bx = 64, by = 2 or 4, occupancy 0 and n % 4 == 0.  The variants read the same stencil values and
+ *    LDS; needs bx = 64, by = 2, 4 or 8, occupancy 0, n % 4 == 0 and
+ *    n <= 544.  The variants read the same stencil values and
  *    agree to FP32 rounding. */
 typedef struct lbm_launch {
     int32_t bx, by, remap, occupancy, variant;

@@ -633,10 +633,11 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes) {
 // Launch variant 1 (LDS-staged z-march) stages rows with 16-byte loads, so
 // rows must be 16-byte aligned (n % 4 == 0); the far-wall node then never
 // sits in the first column or row of a tile and its one-sided stencil stays
-// inside the staged halo.
+// inside the staged halo.  Its staging offsets are 32-bit element offsets
+// over the 26 planes (26 n^3 < 2^32: n <= 544).
 static bool zm_launch_ok(const lbm_ctx &c, int bx, int by, int occ) {
-    return c.dim == 3 && !c.fp64 && bx == 64 && (by == 2 || by == 4) && occ == 0 &&
-           c.n % 4 == 0;
+    return c.dim == 3 && !c.fp64 && bx == 64 && (by == 2 || by == 4 || by == 8) && occ == 0 &&
+           c.n % 4 == 0 && c.n <= 544;
 }
 
 int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
@@ -648,8 +649,8 @@ int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
         return fail(LBM_ERR_ARG, "invalid launch configuration");
     if (l->variant == 1 && !zm_launch_ok(*h, l->bx, l->by, l->occupancy))
         return fail(LBM_ERR_ARG,
-                    "variant 1 needs 3D FP32 storage, bx = 64, by = 2 or 4, occupancy 0 "
-                    "and n % 4 == 0");
+                    "variant 1 needs 3D FP32 storage, bx = 64, by = 2, 4 or 8, occupancy 0, "
+                    "n % 4 == 0 and n <= 544");
     h->shape.bx = l->bx;
     h->shape.by = l->by;
     h->shape.remap = l->remap;

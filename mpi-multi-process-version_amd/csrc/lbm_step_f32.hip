@@ -51,7 +51,7 @@ int zm_planes_per_group(int n, int by) {
 
 hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
                           hipStream_t s) {
-    if (ls.bx != 64 || (ls.by != 2 && ls.by != 4)) return hipErrorInvalidValue;
+    if (ls.bx != 64 || (ls.by != 2 && ls.by != 4 && ls.by != 8)) return hipErrorInvalidValue;
     StepArgs<float> a = a0;
     const int kc = zm_planes_per_group(a.n, ls.by);
     a.gx = (a.n + 63) / 64;
@@ -62,6 +62,9 @@ hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchSha
     if (ls.by == 2) {
         if (macro) hipLaunchKernelGGL((k_step_zm<2, true>), grid, block, 0, s, a, kc);
         else hipLaunchKernelGGL((k_step_zm<2, false>), grid, block, 0, s, a, kc);
+    } else if (ls.by == 8) {
+        if (macro) hipLaunchKernelGGL((k_step_zm<8, true>), grid, block, 0, s, a, kc);
+        else hipLaunchKernelGGL((k_step_zm<8, false>), grid, block, 0, s, a, kc);
     } else {
         if (macro) hipLaunchKernelGGL((k_step_zm<4, true>), grid, block, 0, s, a, kc);
         else hipLaunchKernelGGL((k_step_zm<4, false>), grid, block, 0, s, a, kc);

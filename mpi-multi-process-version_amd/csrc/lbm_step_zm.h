@@ -21,10 +21,14 @@
 // with it to FP32 rounding (the compiler fuses a different handful of
 // multiply-adds in the two kernels).
 //
+// All waves run one copy of the loop: each thread's staging offsets are
+// computed once (a plan) and the loads and LDS writes of a plane are
+// straight-line code.
+//
 // Preconditions (checked by the host): FP32 storage, 3D, bx = 64, by = BY,
 // n % 4 == 0 (16-byte aligned rows; the far-wall node then never sits in
 // the first column or row of a tile, so its one-sided stencil stays inside
-// the staged halo).
+// the staged halo) and n <= 544 (32-bit element offsets over 26 planes).
 #pragma once
 
 #include "lbm_step.h"
@@ -63,124 +67,124 @@ template <int BY> struct ZmLayout {
         return o;
     }
     static constexpr int total() { return off(NP); }
-    // Row quads of a phase (Z: the c_z != 0 populations at a new plane;
-    // !Z: the in-plane populations at plane k): quad m covers rows
-    // 4 qd .. 4 qd + 3 of one population (rows past H - 1 repeat row H - 1).
-    static constexpr int nquads(bool z) {
+    // Rows of a phase (Z: the c_z != 0 populations at a new plane; !Z: the
+    // in-plane populations at plane k), in slot order.  Staging instruction
+    // q of wave w gives lane group g (16 lanes of 4 x values each) row
+    // R = 4 (q BY + w) + g, clamped to the last row (a repeated row writes
+    // the same values).
+    static constexpr int nrows(bool z) {
         int c = 0;
         for (int p = 0; p < NP; p++)
-            if (staged(p) && zpop(p) == z) c += (H(p) + 3) / 4;
+            if (staged(p) && zpop(p) == z) c += H(p);
         return c;
     }
-    static constexpr int quad_pop(bool z, int m) {
-        for (int p = 0; p < NP; p++)
-            if (staged(p) && zpop(p) == z) {
-                const int nq = (H(p) + 3) / 4;
-                if (m < nq) return p;
-                m -= nq;
-            }
-        return -1;
-    }
-    static constexpr int quad_row0(bool z, int m) {
-        for (int p = 0; p < NP; p++)
-            if (staged(p) && zpop(p) == z) {
-                const int nq = (H(p) + 3) / 4;
-                if (m < nq) return 4 * m;
-                m -= nq;
-            }
-        return -1;
-    }
-    // quads of wave W: m = W, W + BY, ...
-    static constexpr int wave_quads(bool z, int W) {
-        return W < nquads(z) ? (nquads(z) - W + BY - 1) / BY : 0;
-    }
+    static constexpr int ninstr(bool z) { return (nrows(z) + 4 * BY - 1) / (4 * BY); }
 };
 
-// Registers holding one wave's quads of one phase while they are in flight,
-// plus the thread's halo value.
-template <int BY, bool Z, int W> struct ZmStage {
+// A thread's staging plan for one phase, fixed for the whole march: for
+// each staging instruction, the element offset of its four values at plane
+// 0 and their LDS index in ring slot 0 (low 16 bits) with the region size
+// (high 16 bits, the ring stride); the same for its halo value, if any
+// (thread t owns halo value t: population, region row, side).
+template <int BY, bool Z> struct ZmPlan {
     using L = ZmLayout<BY>;
-    zm_f4 quad[L::wave_quads(Z, W) > 0 ? L::wave_quads(Z, W) : 1];
-    float halo;
-};
-
-// Per-thread halo assignment of a phase, fixed for the whole march: thread t
-// owns halo value t (population, region row, side), or none.
-struct ZmHalo {
-    size_t goff;     // element offset from the buffer start, without the plane
-    int lds;         // LDS index in ring slot 0, -1 if the thread owns none
-    int psz;         // region size of its population (ring stride)
+    uint32_t goff[L::ninstr(Z)];
+    uint32_t lds[L::ninstr(Z)];
+    uint32_t hoff;
+    uint32_t hlds;   // ~0u: no halo value
 };
 
 template <int BY, bool Z>
-__device__ __forceinline__ ZmHalo zm_halo_plan(const StepArgs<float> &a, int i0, int j0,
-                                               int t) {
+__device__ __forceinline__ ZmPlan<BY, Z> zm_plan(const StepArgs<float> &a, int i0, int j0,
+                                                 int w, int l) {
     using L = ZmLayout<BY>;
     const int n = a.n;
-    ZmHalo z{0, -1, 0};
-    int h = t;
+    const uint32_t plane = (uint32_t)a.plane;
+    const int g = l >> 4, c4 = 4 * (l & 15);
+    const int ii = min(i0 + c4, n - 4);   // n % 4 == 0: groups never straddle the edge
+    ZmPlan<BY, Z> pl;
+    static_for<0, L::ninstr(Z)>([&](auto Q) {
+        constexpr int q = decltype(Q)::value;
+        const int R = min(4 * (q * BY + w) + g, L::nrows(Z) - 1);
+        uint32_t go = 0, ld = 0;
+        int cum = 0;
+        static_for<0, L::NP>([&](auto Pp) {
+            constexpr int p = decltype(Pp)::value;
+            if constexpr (L::staged(p) && L::zpop(p) == Z) {
+                if (R >= cum && R < cum + L::H(p)) {
+                    const int rr = R - cum;
+                    const int jj = min(max(j0 - L::hy(p) + rr, 0), n - 1);
+                    go = (uint32_t)p * plane + (uint32_t)jj * n + ii;
+                    ld = (uint32_t)(L::off(p) + rr * L::P(p) + L::C0(p) + c4) |
+                         ((uint32_t)L::psize(p) << 16);
+                }
+                cum += L::H(p);
+            }
+        });
+        pl.goff[q] = go;
+        pl.lds[q] = ld;
+    });
+    pl.hoff = 0;
+    pl.hlds = ~0u;
+    int h = l + 64 * w;
     static_for<0, L::NP>([&](auto Pp) {
         constexpr int p = decltype(Pp)::value;
         if constexpr (L::staged(p) && L::zpop(p) == Z && L::hx(p) == 1) {
             constexpr int cnt = 2 * L::H(p);
             const bool mine = h >= 0 && h < cnt;
             const int rr = h >> 1, side = h & 1;
-            const int jj = j0 - L::hy(p) + rr, ii = side ? i0 + 64 : i0 - 1;
-            const bool inside = jj >= 0 && jj < n && ii >= 0 && ii < n;
+            const int jj = j0 - L::hy(p) + rr, ih = side ? i0 + 64 : i0 - 1;
+            const bool inside = jj >= 0 && jj < n && ih >= 0 && ih < n;
             if (mine && inside) {
-                z.goff = (size_t)p * (size_t)a.plane + (size_t)jj * n + ii;
-                z.lds = L::off(p) + rr * L::P(p) + (side ? 68 : 3);
-                z.psz = L::psize(p);
+                pl.hoff = (uint32_t)p * plane + (uint32_t)jj * n + ih;
+                pl.hlds = (uint32_t)(L::off(p) + rr * L::P(p) + (side ? 68 : 3)) |
+                          ((uint32_t)L::psize(p) << 16);
             }
             h -= cnt;
         }
     });
-    return z;
+    return pl;
 }
 
-// Loads wave W's quads of phase Z at plane kk (clamped to the grid) into
-// registers.  Straight-line code: every load is issued, so the waits before
-// the compute count them as in flight.
-template <int BY, bool Z, int W>
-__device__ __forceinline__ void zm_fetch(const StepArgs<float> &a, int kk, int i0, int j0,
-                                         const ZmHalo &hp, ZmStage<BY, Z, W> &s) {
+// Registers holding a thread's staged values of one phase while in flight.
+template <int BY, bool Z> struct ZmStage {
     using L = ZmLayout<BY>;
-    const int n = a.n;
-    const int l = threadIdx.x, rsub = l >> 4, c4 = 4 * (l & 15);
-    kk = min(kk, n - 1);
-    const uint32_t pl = (uint32_t)kk * (uint32_t)n * (uint32_t)n;
-    const int ii = min(i0 + c4, n - 4);   // n % 4 == 0: groups never straddle the edge
-    static_for<0, L::wave_quads(Z, W)>([&](auto Q) {
+    zm_f4 quad[L::ninstr(Z)];
+    float halo;
+};
+
+// Loads the thread's values of phase Z at plane kk (clamped to the grid)
+// into registers.  Straight-line code: every load is issued, so the waits
+// before the compute count them as in flight.
+template <int BY, bool Z>
+__device__ __forceinline__ void zm_fetch(const StepArgs<float> &a, int kk,
+                                         const ZmPlan<BY, Z> &pl, ZmStage<BY, Z> &s) {
+    using L = ZmLayout<BY>;
+    kk = min(kk, a.n - 1);
+    const size_t po = (size_t)kk * (size_t)a.n * (size_t)a.n;
+    static_for<0, L::ninstr(Z)>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
-        constexpr int m = W + BY * q;
-        constexpr int p = L::quad_pop(Z, m), row0 = L::quad_row0(Z, m);
-        const int rr = min(row0 + rsub, L::H(p) - 1);
-        const int jj = min(max(j0 - L::hy(p) + rr, 0), n - 1);
-        s.quad[q] = *reinterpret_cast<const zm_f4 *>(
-            reinterpret_cast<const char *>(a.pin + p * a.plane) +
-            (pl + (uint32_t)jj * n + ii) * (uint32_t)sizeof(float));
+        s.quad[q] = *reinterpret_cast<const zm_f4 *>(a.pin + (po + pl.goff[q]));
     });
-    // threads without a halo value load their own plane origin (unused)
-    s.halo = a.pin[(size_t)pl + hp.goff];
+    // threads without a halo value load the plane origin (unused)
+    s.halo = a.pin[po + pl.hoff];
 }
 
-// Writes fetched quads into their LDS slot (Z: ring slot of the plane).
-template <int BY, bool Z, int W>
-__device__ __forceinline__ void zm_commit(float *lds, const ZmHalo &hp,
-                                          const ZmStage<BY, Z, W> &s, int slot) {
+// Writes fetched values into their LDS slot (Z: ring slot of the plane),
+// as two 8-byte writes per group of four (fewer bank conflicts than one
+// 16-byte write).
+template <int BY, bool Z>
+__device__ __forceinline__ void zm_commit(float *lds, const ZmPlan<BY, Z> &pl,
+                                          const ZmStage<BY, Z> &s, int slot) {
     using L = ZmLayout<BY>;
-    const int l = threadIdx.x, rsub = l >> 4, c4 = 4 * (l & 15);
-    static_for<0, L::wave_quads(Z, W)>([&](auto Q) {
+    static_for<0, L::ninstr(Z)>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
-        constexpr int m = W + BY * q;
-        constexpr int p = L::quad_pop(Z, m), row0 = L::quad_row0(Z, m);
-        const int rr = min(row0 + rsub, L::H(p) - 1);
-        // two 8-byte writes: fewer LDS bank conflicts than one 16-byte write
-        float *dst = &lds[L::off(p) + L::psize(p) * (Z ? slot : 0) + rr * L::P(p) + L::C0(p) + c4];
-        *reinterpret_cast<zm_f2 *>(dst) = s.quad[q].xy;
-        *reinterpret_cast<zm_f2 *>(dst + 2) = s.quad[q].zw;
+        const uint32_t idx = (pl.lds[q] & 0xffffu) + (Z ? (uint32_t)slot * (pl.lds[q] >> 16) : 0u);
+        *reinterpret_cast<zm_f2 *>(&lds[idx]) = s.quad[q].xy;
+        *reinterpret_cast<zm_f2 *>(&lds[idx + 2]) = s.quad[q].zw;
     });
-    if (hp.lds >= 0) lds[hp.lds + hp.psz * (Z ? slot : 0)] = s.halo;
+    if (pl.hlds != ~0u)
+        lds[(pl.hlds & 0xffffu) + (Z ? (uint32_t)slot * (pl.hlds >> 16) : 0u)] = s.halo;
 }
 
 // Value of slot p at x, y offsets (dx, dy) from the lane's node, in ring
@@ -242,17 +246,21 @@ __device__ __forceinline__ float zm_interp(const float *lds, int l, int r, float
 
 __device__ __forceinline__ int zm_window(int k, int n) { return min(max(k - 1, 0), n - 3); }
 
-// The march of wave W of a workgroup (the caller branches on the wave index
-// once, outside the loop, so each wave's loop is straight-line code).
-template <int BY, bool MACRO, int W>
-__device__ __forceinline__ void zm_march(const StepArgs<float> &a, float *lds, int i0, int j0,
-                                         int k0, int k1) {
+template <int BY, bool MACRO>
+__global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) {
     using L = ZmLayout<BY>;
     using FL = Flow<3>;
     using HT = Heat<3>;
     constexpr int QF = FL::Q, QG = HT::Q;
+    __shared__ zm_f4 lds4[L::total() / 4];
+    float *lds = reinterpret_cast<float *>(lds4);
     const int n = a.n;
-    const int l = threadIdx.x, r = W;
+    int tx, ty, tc;
+    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tc);
+    const int i0 = tx * 64, j0 = ty * BY;
+    const int k0 = tc * kc, k1 = min(n, k0 + kc);
+    const int l = threadIdx.x;
+    const int r = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one row per wave
     const bool live = i0 + l < n && j0 + r < n;
     // lanes past the grid edge stage and compute like the last node and
     // store nothing (every lane must reach the barriers)
@@ -260,19 +268,19 @@ __device__ __forceinline__ void zm_march(const StepArgs<float> &a, float *lds, i
     AxisStencil<float> st[3];
     st[0] = load_stencil(a, i);
     st[1] = load_stencil(a, j);
-    const ZmHalo hin = zm_halo_plan<BY, false>(a, i0, j0, l + 64 * W);
-    const ZmHalo hz = zm_halo_plan<BY, true>(a, i0, j0, l + 64 * W);
+    const ZmPlan<BY, false> pin_ = zm_plan<BY, false>(a, i0, j0, r, l);
+    const ZmPlan<BY, true> pz = zm_plan<BY, true>(a, i0, j0, r, l);
 
     {   // prologue: the z window of plane k0 and the in-plane rows of k0
         const int ws = zm_window(k0, n);
         for (int zz = ws; zz < ws + 3; zz++) {
-            ZmStage<BY, true, W> sz;
-            zm_fetch<BY, true, W>(a, zz, i0, j0, hz, sz);
-            zm_commit<BY, true, W>(lds, hz, sz, zz % 3);
+            ZmStage<BY, true> sz;
+            zm_fetch<BY, true>(a, zz, pz, sz);
+            zm_commit<BY, true>(lds, pz, sz, zz % 3);
         }
-        ZmStage<BY, false, W> sp;
-        zm_fetch<BY, false, W>(a, k0, i0, j0, hin, sp);
-        zm_commit<BY, false, W>(lds, hin, sp, 0);
+        ZmStage<BY, false> sp;
+        zm_fetch<BY, false>(a, k0, pin_, sp);
+        zm_commit<BY, false>(lds, pin_, sp, 0);
     }
     __syncthreads();
 
@@ -296,10 +304,10 @@ __device__ __forceinline__ void zm_march(const StepArgs<float> &a, float *lds, i
         pcf[0] = ldp(a.pin, node);
         pcg[0] = ldp(a.pin + QF * a.plane, node);
         __builtin_amdgcn_sched_barrier(0);
-        ZmStage<BY, false, W> np;
-        ZmStage<BY, true, W> nzs;
-        zm_fetch<BY, false, W>(a, kn, i0, j0, hin, np);
-        zm_fetch<BY, true, W>(a, wsn + 2, i0, j0, hz, nzs);
+        ZmStage<BY, false> np;
+        ZmStage<BY, true> nzs;
+        zm_fetch<BY, false>(a, kn, pin_, np);
+        zm_fetch<BY, true>(a, wsn + 2, pz, nzs);
         __builtin_amdgcn_sched_barrier(0);
 
         static_for<1, L::NP>([&](auto Pp) {
@@ -327,27 +335,10 @@ __device__ __forceinline__ void zm_march(const StepArgs<float> &a, float *lds, i
             node_tail<3, float, float, true, MACRO>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g, hs);
 
         __syncthreads();   // every read of plane k's staging is done
-        if (more) zm_commit<BY, false, W>(lds, hin, np, 0);
-        if (newz) zm_commit<BY, true, W>(lds, hz, nzs, (wsn + 2) % 3);
+        if (more) zm_commit<BY, false>(lds, pin_, np, 0);
+        if (newz) zm_commit<BY, true>(lds, pz, nzs, (wsn + 2) % 3);
         __syncthreads();
     }
-}
-
-template <int BY, bool MACRO>
-__global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) {
-    using L = ZmLayout<BY>;
-    __shared__ zm_f4 lds4[L::total() / 4];
-    float *lds = reinterpret_cast<float *>(lds4);
-    const int n = a.n;
-    int tx, ty, tc;
-    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tc);
-    const int i0 = tx * 64, j0 = ty * BY;
-    const int k0 = tc * kc, k1 = min(n, k0 + kc);
-    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    static_for<0, BY>([&](auto WW) {
-        constexpr int W = decltype(WW)::value;
-        if (w == W) zm_march<BY, MACRO, W>(a, lds, i0, j0, k0, k1);
-    });
 }
 
 }  // namespace lbm

@@ -261,6 +261,8 @@ def test_tile_orders_give_identical_results(hip, dim, n, bx, by):
     (72, 2, 2),
     (96, 4, 0),
     (132, 4, 2),    # y bands per XCD
+    (24, 8, 0),
+    (136, 8, 2),
 ])
 def test_lds_zmarch_variant_gives_identical_results(hip, n, by, remap):
     """Variant 1 stages the populations in LDS and marches in z; its
@@ -315,7 +317,7 @@ def test_lds_zmarch_variant_is_rejected_where_it_cannot_run(hip):
     cases = [(dict(dim=2, n=64), (64, 4)),            # 3D only
              (dict(dim=3, n=64, precision=hip.PREC_FP64), (64, 4)),
              (dict(dim=3, n=64), (128, 2)),          # bx must be 64
-             (dict(dim=3, n=64), (64, 8)),           # by 2 or 4
+             (dict(dim=3, n=64), (64, 16)),          # by 2, 4 or 8
              (dict(dim=3, n=66), (64, 4)),           # rows not 16-byte aligned
              (dict(dim=3, n=65), (64, 2))]
     for kw, (bx, by) in cases:
