@@ -29,15 +29,17 @@ WORKLOADS = {
 }
 
 
-def traffic_from_profiles(workload):
+def traffic_from_profiles(workload, precision, launch):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE, scaled by the copy-kernel calibration), if a summary for
-    this workload has been committed under profiles/."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    WRITE_SIZE, scaled by the copy-kernel calibration), if a summary for this
+    workload, precision and launch shape has been committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{precision}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("bytes_per_launch")
+        d = json.load(fh)
+    shape = ",".join(str(launch[k]) for k in ("bx", "by", "remap", "occupancy", "variant"))
+    return d.get("bytes_per_launch") if d.get("launch") == shape else None
 
 
 def cpu_baseline(cfg, threads, sample_steps):
@@ -118,7 +120,7 @@ def main():
     nu = s.nusselt()
     s.close()
 
-    traffic = traffic_from_profiles(args.workload)
+    traffic = traffic_from_profiles(args.workload, args.precision, launch)
     out = {
         "metric": "MLUPS + % HBM peak (roofline.frac), fused MRT-LBM step, 1 GPU",
         "value": round(mlups, 1),
