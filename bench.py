@@ -26,7 +26,19 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 WORKLOADS = {
     "3d256": dict(dim=3, n=256, Ra=1e5, label="cavity3d_256_ra1e5_d3q19_d3q7"),
     "2d1025": dict(dim=2, n=1025, Ra=1e6, label="cavity2d_1025_ra1e6_d2q9_d2q5"),
+    "batch64": dict(dim=2, count=64, seed=20261203,
+                    label="batch_64_cavities2d_n65-513_A1-2.5_Ra1e3-1e6_Pr0.7-7"),
 }
+
+
+def batch_cavities(count, seed):
+    """The batched sweep's cavities: edge from {65, 129, 257, 513}, erf
+    stretch factor A ~ U[1, 2.5], log10 Ra ~ U[3, 6], Pr ~ U[0.7, 7]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [dict(n=int(rng.choice([65, 129, 257, 513])), stretch=float(rng.uniform(1.0, 2.5)),
+                 Ra=float(10.0 ** rng.uniform(3.0, 6.0)), Pr=float(rng.uniform(0.7, 7.0)))
+            for _ in range(count)]
 
 
 def traffic_from_profiles(workload, precision, launch):
@@ -58,6 +70,80 @@ def cpu_baseline(cfg, threads, sample_steps):
             "cores": threads, "kind": "port",
             "sample": f"{sample_steps} steps of the same {cfg['n']}^{cfg['dim']} cavity "
                       f"(FP64 CPU oracle, {threads} OpenMP threads, {dt:.1f} s)"}
+
+
+def run_batch(args, cfg):
+    """The batched sweep: every cavity advanced by one launch per step."""
+    import numpy as np
+    import lbmhip
+    cavs = batch_cavities(cfg["count"], cfg["seed"])
+    b = lbmhip.Batch(cavs, device=0)
+    b.init_rest()
+    if args.warmup:
+        b.step(args.warmup)
+    if not args.no_autotune:
+        b.autotune(5)   # restores the state it started from
+    launch = b.get_launch()
+    _, nodes = b.active()
+    t0 = time.perf_counter()
+    ms = b.time_steps(args.steps)
+    wall = time.perf_counter() - t0
+    sec = ms / 1e3
+    mlups = nodes * args.steps / sec / 1e6
+    bpu = 2 * 4 * (9 + 5)
+    achieved = bpu * nodes / (sec / args.steps) / 1e9
+    # per-cavity Nusselt numbers against the FP64 path after the same steps
+    nu = np.array(b.nusselt()[0])
+    errs = []
+    for i, c in enumerate(cavs):
+        s = lbmhip.Solver(2, c["n"], c["Ra"], Pr=c["Pr"], stretch=c["stretch"],
+                          precision=lbmhip.PREC_FP64)
+        s.init_rest()
+        s.step(b[i].itc)
+        ref = s.nusselt()[0]
+        errs.append(abs(nu[i] - ref) / abs(ref))
+        s.close()
+    errs = np.array(errs)
+    sizes = {k: sum(1 for c in cavs if c["n"] == k) for k in (65, 129, 257, 513)}
+    out = {
+        "metric": "MLUPS (aggregate over the batch) + % HBM peak (roofline.frac), 1 GPU",
+        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded cavity parameters, reference initial condition",
+        "config": {"workload": cfg["label"], "cavities": len(cavs), "nodes": nodes,
+                   "edges": sizes, "seed": cfg["seed"], "launch": launch,
+                   "parallelism": "single GPU, one launch per step for all cavities"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes_per_launch": bpu * nodes},
+        "wall_s": round(wall, 4),
+        "check": {"itc": int(b[0].itc), "nu_rel_err_vs_fp64_max": float(errs.max()),
+                  "nu_rel_err_vs_fp64_median": float(np.median(errs))},
+    }
+    b.close()
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import lbm_oracle
+        lbm_oracle.set_threads(args.cpu_threads)
+        steps = args.cpu_steps or 20
+        t = 0.0
+        for c in cavs:
+            o = lbm_oracle.Oracle(2, c["n"], c["Ra"], Pr=c["Pr"], A=c["stretch"])
+            o.init_rest()
+            o.step(1)
+            t0 = time.perf_counter()
+            o.step(steps)
+            t += time.perf_counter() - t0
+            o.close()
+        out["cpu_baseline"] = {
+            "value": nodes * steps / t / 1e6, "unit": "MLUPS", "cores": args.cpu_threads,
+            "kind": "port",
+            "sample": f"{steps} steps of each of the {len(cavs)} cavities, one after another "
+                      f"(FP64 CPU oracle, {args.cpu_threads} OpenMP threads, {t:.1f} s)"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -94,6 +180,13 @@ def main():
 
     import lbmhip
     cfg = WORKLOADS[args.workload]
+    if "count" in cfg:
+        if args.precision != "fp32" or args.launch:
+            print("bench.py: the batch workload runs FP32 with the batch autotuner",
+                  file=sys.stderr)
+            sys.exit(2)
+        run_batch(args, cfg)
+        return
     prec = lbmhip.PREC_FP64 if args.precision == "fp64" else lbmhip.PREC_FP32_SHIFTED
     s = lbmhip.Solver(cfg["dim"], cfg["n"], cfg["Ra"], precision=prec, device=0)
     s.init_rest()

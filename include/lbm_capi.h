@@ -200,6 +200,66 @@ int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms);
  * roofline: every population read once and written once. */
 int32_t lbm_bytes_per_update(lbm_handle h, double *bytes);
 
+/* ------------------------------------------------------------------------
+ * Batches of independent 2D cavities (no reference counterpart: the
+ * reference runs one cavity per program; each cavity of a batch follows the
+ * single-cavity semantics above, per cavity, natural-convection_2dim_mono.f90
+ * throughout).  One kernel launch per time step advances every active cavity
+ * of the batch, whatever their sizes, grids and parameters: workgroups are
+ * dealt tiles of all cavities from one table.  2D, FP32 shifted storage, one
+ * device.  Arrays of per-cavity results hold `count` values in batch order.
+ */
+typedef struct lbm_batch_ctx *lbm_batch;
+
+/* Creates `count` cavities from params[0..count-1] (each as lbm_create;
+ * all 2D, precision LBM_PREC_FP32_SHIFTED, the same device and the same
+ * check_every). */
+int32_t lbm_batch_create(const lbm_params *params, int32_t count, lbm_batch *out);
+int32_t lbm_batch_destroy(lbm_batch b);
+
+/* The handle of cavity `index`, owned by the batch.  Use it for the
+ * read-only calls (lbm_get_grid, lbm_get_info, lbm_get_fields, lbm_nusselt,
+ * lbm_get_itc, lbm_state_size, lbm_save_state) and to initialise one cavity
+ * (lbm_init_rest, lbm_init_fields, lbm_load_state); step, check and run
+ * through the batch. */
+lbm_handle lbm_batch_cavity(lbm_batch b, int32_t index);
+
+/* lbm_init_rest on every cavity; all cavities become active. */
+int32_t lbm_batch_init_rest(lbm_batch b);
+
+/* nsteps time steps of every active cavity (asynchronous). */
+int32_t lbm_batch_step(lbm_batch b, int64_t nsteps);
+
+/* lbm_check on every cavity (errU, errT: count values each). */
+int32_t lbm_batch_check(lbm_batch b, double *errU, double *errT);
+
+/* lbm_nusselt on every cavity. */
+int32_t lbm_batch_nusselt(lbm_batch b, double *nu_hot, double *nu_cold);
+
+/* Runs every cavity with the stopping rule of lbm_run, each with its own
+ * epsU, epsT and itc_max: the active cavities step together in windows of
+ * check_every; a cavity whose residuals both reach its eps, whose itc
+ * reaches its itc_max, or whose residual is non-finite leaves the batch.
+ * itc, errU, errT, status (LBM_OK, LBM_ERR_NOT_CONVERGED or
+ * LBM_ERR_DIVERGED): count values each, any may be NULL.  Returns LBM_OK
+ * when the run completed, whatever the per-cavity statuses. */
+int32_t lbm_batch_run(lbm_batch b, int64_t *itc, double *errU, double *errT, int32_t *status);
+
+/* Number of active cavities and their total node count (the MLUPS
+ * numerator of one step). */
+int32_t lbm_batch_active(lbm_batch b, int32_t *cavities, int64_t *nodes);
+
+/* lbm_time_steps for the batch: nsteps batched steps between two HIP events
+ * on the batch's stream, in ms. */
+int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms);
+
+/* Block shape of the batched launch (bx a multiple of 32, bx*by a multiple
+ * of 64 and at most 1024; remap, occupancy and variant must be 0), and its
+ * autotuner (restores every cavity's state). */
+int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *launch);
+int32_t lbm_batch_get_launch(lbm_batch b, lbm_launch *launch);
+int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms);
+
 const char *lbm_status_string(int32_t status);
 const char *lbm_last_error(void);
 

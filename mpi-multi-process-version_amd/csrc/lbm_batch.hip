@@ -1,0 +1,399 @@
+// lbm_batch.hip -- batches of independent 2D cavities behind the C ABI
+// (lbm_batch_* in include/lbm_capi.h).
+//
+// Each cavity is a full single-cavity context (grid, tables, buffers); the
+// batch owns them and one HIP stream that all of them use.  A time step of
+// the batch is one launch of k_step_batch over a table of tiles of every
+// active cavity: workgroup b takes tile b, reads its cavity's step
+// arguments from a device array (two per cavity, one per ping-pong parity)
+// and runs the single-cavity node update on it.  Tiles are listed largest
+// cavity first, so the big cavities start early and the small ones fill
+// the tail of the launch.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "lbm_internal.h"
+
+using lbm::set_error;
+
+struct lbm_batch_ctx {
+    std::vector<lbm_handle> cav;
+    std::vector<char> active;
+    std::vector<int> base_cur;   // cur of each cavity when the tile table was built
+    int device = 0;
+    int check_every = 2000;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    lbm::StepArgs<float> *d_args = nullptr;   // [2c + p]: cavity c reading buffer p
+    int4 *d_tiles = nullptr;
+    int ntiles = 0, tiles_cap = 0;
+    int parity = 0;   // launches since the table was built, mod 2
+    bool dirty = true;
+    int bx = 64, by = 4;
+};
+
+namespace {
+
+int32_t hip_err(hipError_t e, const char *where) {
+    return set_error(LBM_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define BATCH_TRY(expr)                                      \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_err(e_, #expr);     \
+    } while (0)
+
+void release(lbm_batch b) {
+    for (lbm_handle h : b->cav) {
+        if (!h) continue;
+        h->stream = nullptr;   // the batch's stream, destroyed below
+        (void)lbm_destroy(h);
+    }
+    if (b->d_args) (void)hipFree(b->d_args);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int32_t check_batch(lbm_batch b) {
+    if (!b) return set_error(LBM_ERR_ARG, "null batch");
+    BATCH_TRY(hipSetDevice(b->device));
+    return LBM_OK;
+}
+
+bool table_stale(const lbm_batch b) {
+    if (b->dirty) return true;
+    for (size_t c = 0; c < b->cav.size(); c++)
+        if (b->active[c] && b->cav[c]->cur != (b->base_cur[c] ^ b->parity)) return true;
+    return false;
+}
+
+// Lists the tiles of every active cavity, largest cavity first.
+int32_t rebuild(lbm_batch b) {
+    BATCH_TRY(hipStreamSynchronize(b->stream));   // the old table may still be read
+    std::vector<int> order(b->cav.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return b->cav[x]->n > b->cav[y]->n; });
+    std::vector<int4> tiles;
+    for (int c : order) {
+        if (!b->active[c]) continue;
+        const lbm_ctx &h = *b->cav[c];
+        b->base_cur[c] = h.cur;
+        const int gx = (h.n + b->bx - 1) / b->bx, gy = (h.n + b->by - 1) / b->by;
+        for (int ty = 0; ty < gy; ty++)
+            for (int tx = 0; tx < gx; tx++) tiles.push_back(make_int4(c, tx, ty, 2 * c + h.cur));
+    }
+    if ((int)tiles.size() > b->tiles_cap) {
+        if (b->d_tiles) BATCH_TRY(hipFree(b->d_tiles));
+        b->d_tiles = nullptr;
+        b->tiles_cap = 0;
+        BATCH_TRY(hipMalloc(&b->d_tiles, sizeof(int4) * tiles.size()));
+        b->tiles_cap = (int)tiles.size();
+    }
+    if (!tiles.empty())
+        BATCH_TRY(hipMemcpy(b->d_tiles, tiles.data(), sizeof(int4) * tiles.size(),
+                            hipMemcpyHostToDevice));
+    b->ntiles = (int)tiles.size();
+    b->parity = 0;
+    b->dirty = false;
+    return LBM_OK;
+}
+
+int32_t batch_steps(lbm_batch b, int64_t nsteps) {
+    if (nsteps <= 0) return LBM_OK;
+    for (size_t c = 0; c < b->cav.size(); c++)
+        if (b->active[c] && !b->cav[c]->initialized)
+            return set_error(LBM_ERR_STATE, "cavity " + std::to_string(c) + " not initialised");
+    if (table_stale(b)) {
+        int32_t rc = rebuild(b);
+        if (rc) return rc;
+    }
+    for (int64_t s = 0; s < nsteps; s++) {
+        hipError_t e = lbm::launch_step_batch_f32(s == nsteps - 1, b->d_args, b->d_tiles,
+                                                  b->ntiles, b->parity, b->bx, b->by, b->stream);
+        if (e != hipSuccess) return hip_err(e, "batched step launch");
+        b->parity ^= 1;
+    }
+    for (size_t c = 0; c < b->cav.size(); c++) {
+        if (!b->active[c]) continue;
+        b->cav[c]->cur ^= (int)(nsteps & 1);
+        b->cav[c]->itc += nsteps;
+    }
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t lbm_batch_create(const lbm_params *params, int32_t count, lbm_batch *out) {
+    if (!out) return set_error(LBM_ERR_ARG, "null output");
+    *out = nullptr;
+    if (!params || count < 1) return set_error(LBM_ERR_ARG, "need at least one cavity");
+    for (int32_t c = 0; c < count; c++) {
+        const lbm_params &p = params[c];
+        if (p.dim != 2 || p.precision != LBM_PREC_FP32_SHIFTED)
+            return set_error(LBM_ERR_ARG, "batched cavities are 2D with FP32 shifted storage");
+        if (p.device != params[0].device || p.check_every != params[0].check_every)
+            return set_error(LBM_ERR_ARG, "batched cavities share the device and check_every");
+    }
+    lbm_batch b = new (std::nothrow) lbm_batch_ctx();
+    if (!b) return set_error(LBM_ERR_NOMEM, "batch");
+    b->device = params[0].device;
+    b->check_every = params[0].check_every;
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        release(b);
+        return hip_err(e, "batch stream");
+    }
+    b->cav.assign(count, nullptr);
+    for (int32_t c = 0; c < count; c++) {
+        int32_t rc = lbm_create(&params[c], &b->cav[c]);
+        if (rc) {
+            const std::string msg = "cavity " + std::to_string(c) + ": " + lbm_last_error();
+            release(b);
+            return set_error(rc, msg);
+        }
+        // every cavity works on the batch's stream
+        (void)hipStreamDestroy(b->cav[c]->stream);
+        b->cav[c]->stream = b->stream;
+    }
+    b->active.assign(count, 0);
+    b->base_cur.assign(count, 0);
+    std::vector<lbm::StepArgs<float>> args(2 * (size_t)count);
+    for (int32_t c = 0; c < count; c++)
+        for (int p = 0; p < 2; p++) {
+            lbm::StepArgs<float> a = b->cav[c]->a32;
+            a.pin = (const float *)b->cav[c]->pbuf[p];
+            a.pout = (float *)b->cav[c]->pbuf[1 - p];
+            args[2 * c + p] = a;
+        }
+    e = hipMalloc(&b->d_args, sizeof(lbm::StepArgs<float>) * args.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_args, args.data(), sizeof(lbm::StepArgs<float>) * args.size(),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        release(b);
+        return hip_err(e, "batch arguments");
+    }
+    *out = b;
+    return LBM_OK;
+}
+
+int32_t lbm_batch_destroy(lbm_batch b) {
+    if (!b) return LBM_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    release(b);
+    return LBM_OK;
+}
+
+lbm_handle lbm_batch_cavity(lbm_batch b, int32_t index) {
+    if (!b || index < 0 || index >= (int32_t)b->cav.size()) {
+        set_error(LBM_ERR_ARG, "cavity index out of range");
+        return nullptr;
+    }
+    return b->cav[index];
+}
+
+int32_t lbm_batch_init_rest(lbm_batch b) {
+    int32_t rc = check_batch(b);
+    if (rc) return rc;
+    for (size_t c = 0; c < b->cav.size(); c++) {
+        if ((rc = lbm_init_rest(b->cav[c]))) return rc;
+        b->active[c] = 1;
+    }
+    b->dirty = true;
+    return LBM_OK;
+}
+
+int32_t lbm_batch_step(lbm_batch b, int64_t nsteps) {
+    int32_t rc = check_batch(b);
+    if (rc) return rc;
+    if (nsteps < 0) return set_error(LBM_ERR_ARG, "nsteps < 0");
+    return batch_steps(b, nsteps);
+}
+
+int32_t lbm_batch_check(lbm_batch b, double *errU, double *errT) {
+    int32_t rc = check_batch(b);
+    if (rc) return rc;
+    for (size_t c = 0; c < b->cav.size(); c++) {
+        double eu = 0.0, et = 0.0;
+        rc = lbm_check(b->cav[c], &eu, &et);
+        if (errU) errU[c] = eu;
+        if (errT) errT[c] = et;
+        if (rc && rc != LBM_ERR_DIVERGED) return rc;
+    }
+    return LBM_OK;
+}
+
+int32_t lbm_batch_nusselt(lbm_batch b, double *nu_hot, double *nu_cold) {
+    int32_t rc = check_batch(b);
+    if (rc) return rc;
+    for (size_t c = 0; c < b->cav.size(); c++) {
+        double h = NAN, k = NAN;
+        rc = lbm_nusselt(b->cav[c], &h, &k);
+        if (nu_hot) nu_hot[c] = h;
+        if (nu_cold) nu_cold[c] = k;
+        if (rc && rc != LBM_ERR_DIVERGED) return rc;
+    }
+    return LBM_OK;
+}
+
+int32_t lbm_batch_run(lbm_batch b, int64_t *itc, double *errU, double *errT, int32_t *status) {
+    int32_t rc = check_batch(b);
+    if (rc) return rc;
+    const size_t nc = b->cav.size();
+    const int64_t every = b->check_every;
+    std::vector<double> eu(nc, 100.0), et(nc, 100.0);   // 2dim_mono.f90:148-149
+    std::vector<int32_t> st(nc, LBM_ERR_NOT_CONVERGED);
+    for (size_t c = 0; c < nc; c++) {
+        if (!b->cav[c]->initialized) return set_error(LBM_ERR_STATE, "cavity not initialised");
+        b->active[c] = b->cav[c]->itc < b->cav[c]->p.itc_max;
+    }
+    b->dirty = true;
+    for (;;) {
+        int64_t n = INT64_MAX;
+        for (size_t c = 0; c < nc; c++) {
+            if (!b->active[c]) continue;
+            const lbm_ctx &h = *b->cav[c];
+            n = std::min<int64_t>(n, std::min<int64_t>(every - h.itc % every, h.p.itc_max - h.itc));
+        }
+        if (n == INT64_MAX) break;   // no active cavity left
+        if ((rc = batch_steps(b, n))) return rc;
+        bool changed = false;
+        for (size_t c = 0; c < nc; c++) {
+            if (!b->active[c]) continue;
+            lbm_ctx &h = *b->cav[c];
+            bool done = false;
+            if (h.itc % every == 0) {
+                rc = lbm_check(&h, &eu[c], &et[c]);
+                if (rc == LBM_ERR_DIVERGED) {
+                    st[c] = LBM_ERR_DIVERGED;
+                    done = true;
+                } else if (rc) {
+                    return rc;
+                } else if (eu[c] <= h.p.epsU && et[c] <= h.p.epsT) {
+                    st[c] = LBM_OK;
+                    done = true;
+                }
+            }
+            if (!done && h.itc >= h.p.itc_max) done = true;   // st stays NOT_CONVERGED
+            if (done) {
+                b->active[c] = 0;
+                changed = true;
+            }
+        }
+        if (changed) b->dirty = true;
+    }
+    BATCH_TRY(hipStreamSynchronize(b->stream));
+    for (size_t c = 0; c < nc; c++) {
+        if (itc) itc[c] = b->cav[c]->itc;
+        if (errU) errU[c] = eu[c];
+        if (errT) errT[c] = et[c];
+        if (status) status[c] = st[c];
+    }
+    return LBM_OK;
+}
+
+int32_t lbm_batch_active(lbm_batch b, int32_t *cavities, int64_t *nodes) {
+    if (!b) return set_error(LBM_ERR_ARG, "null batch");
+    int32_t k = 0;
+    int64_t m = 0;
+    for (size_t c = 0; c < b->cav.size(); c++)
+        if (b->active[c]) {
+            k++;
+            m += b->cav[c]->nn;
+        }
+    if (cavities) *cavities = k;
+    if (nodes) *nodes = m;
+    return LBM_OK;
+}
+
+int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms) {
+    int32_t rc = check_batch(b);
+    if (rc) return rc;
+    if (table_stale(b) && (rc = rebuild(b))) return rc;   // the timed region only launches
+    BATCH_TRY(hipEventRecord(b->ev0, b->stream));
+    if ((rc = batch_steps(b, nsteps))) return rc;
+    BATCH_TRY(hipEventRecord(b->ev1, b->stream));
+    BATCH_TRY(hipEventSynchronize(b->ev1));
+    float t = 0.0f;
+    BATCH_TRY(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    if (ms) *ms = t;
+    return LBM_OK;
+}
+
+int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *l) {
+    if (!b || !l) return set_error(LBM_ERR_ARG, "null argument");
+    if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
+        l->remap != 0 || l->occupancy != 0 || l->variant != 0)
+        return set_error(LBM_ERR_ARG, "invalid batched launch configuration");
+    b->bx = l->bx;
+    b->by = l->by;
+    b->dirty = true;
+    return LBM_OK;
+}
+
+int32_t lbm_batch_get_launch(lbm_batch b, lbm_launch *l) {
+    if (!b || !l) return set_error(LBM_ERR_ARG, "null argument");
+    *l = lbm_launch{b->bx, b->by, 0, 0, 0};
+    return LBM_OK;
+}
+
+int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms) {
+    int32_t rc = check_batch(b);
+    if (rc) return rc;
+    if (steps_per_trial < 1) return set_error(LBM_ERR_ARG, "steps_per_trial < 1");
+    const size_t nc = b->cav.size();
+    // host snapshot of every initialised cavity
+    std::vector<std::vector<char>> saved(nc);
+    for (size_t c = 0; c < nc; c++) {
+        if (!b->cav[c]->initialized) continue;
+        int64_t bytes = 0;
+        if ((rc = lbm_state_size(b->cav[c], &bytes))) return rc;
+        saved[c].resize((size_t)bytes);
+        if ((rc = lbm_save_state(b->cav[c], saved[c].data(), bytes))) return rc;
+    }
+    static const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {64, 8}, {128, 1},
+                                  {128, 2}, {128, 4}, {256, 1}, {256, 2}, {32, 8}};
+    const int bx0 = b->bx, by0 = b->by;
+    int best_bx = bx0, best_by = by0;
+    double best = 1e300;
+    for (const auto &cd : cand) {
+        b->bx = cd[0];
+        b->by = cd[1];
+        b->dirty = true;
+        double ms = 0.0;
+        if ((rc = batch_steps(b, 1)) || (rc = lbm_batch_time_steps(b, steps_per_trial, &ms)))
+            break;
+        if (ms / steps_per_trial < best) {
+            best = ms / steps_per_trial;
+            best_bx = cd[0];
+            best_by = cd[1];
+        }
+    }
+    b->bx = rc ? bx0 : best_bx;
+    b->by = rc ? by0 : best_by;
+    b->dirty = true;
+    BATCH_TRY(hipStreamSynchronize(b->stream));
+    for (size_t c = 0; c < nc; c++) {
+        if (saved[c].empty()) continue;
+        int32_t r2 = lbm_load_state(b->cav[c], saved[c].data(), (int64_t)saved[c].size());
+        if (r2 && !rc) rc = r2;
+    }
+    if (rc) return rc;
+    if (best_ms) *best_ms = best;
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -18,8 +18,6 @@
 
 using lbm::Ctx;
 
-struct lbm_ctx : public Ctx {};
-
 namespace {
 
 thread_local std::string g_err;
@@ -230,6 +228,10 @@ int32_t do_steps(Ctx &c, int64_t nsteps) {
 }
 
 }  // namespace
+
+namespace lbm {
+int32_t set_error(int32_t code, const std::string &msg) { return fail(code, msg); }
+}  // namespace lbm
 
 extern "C" {
 

@@ -76,7 +76,19 @@ hipError_t launch_init_rest(const Ctx &c);
 hipError_t launch_residual(const Ctx &c);   // results: d_red[0..3]
 hipError_t launch_nusselt(const Ctx &c);    // results: d_red[4..5]
 
+// Batched step (lbm_step_f32.hip): one launch advances every cavity listed
+// in the tile table.  tiles[b] = (cavity, tile x, tile y, argument index of
+// the cavity at parity 0); workgroup b uses args[tiles[b].w ^ parity].  2D,
+// FP32 shifted storage.
+hipError_t launch_step_batch_f32(bool macro, const StepArgs<float> *args, const int4 *tiles,
+                                 int ntiles, int parity, int bx, int by, hipStream_t s);
+
 constexpr int kRedThreads = 256;
 constexpr int kRedMaxBlocks = 1024;
 
+// Records the detail message returned by lbm_last_error() and returns code.
+int32_t set_error(int32_t code, const std::string &msg);
+
 }  // namespace lbm
+
+struct lbm_ctx : public lbm::Ctx {};

@@ -676,20 +676,14 @@ __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node
 // WPE > 0 asks the compiler for at least WPE resident waves per SIMD (it
 // then caps the registers per lane), trading load hoisting for latency
 // hiding; the autotuner picks.
-template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, int WPE = 0>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
-void k_step(StepArgs<R, S> a) {
+// One node (i, j, k) of the fused step (the node must be inside the grid).
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO>
+__device__ __forceinline__ void step_node(const StepArgs<R, S> &a, int i, int j, int k) {
     constexpr bool EXACT = std::is_same<S, double>::value;
     using FL = Flow<DIM>;
     using HT = Heat<DIM>;
     constexpr int QF = FL::Q, QG = HT::Q;
     const int n = a.n;
-    int tx, ty, tz;
-    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tz);
-    const int i = tx * blockDim.x + threadIdx.x;
-    const int j = ty * blockDim.y + threadIdx.y;
-    const int k = DIM == 3 ? tz : 0;
-    if (i >= n || j >= n) return;
     const uint32_t sy = n, sz = (uint32_t)n * n;
     const uint32_t node = i + sy * j + sz * k;
     const int idx[3] = {i, j, k};
@@ -725,6 +719,18 @@ void k_step(StepArgs<R, S> a) {
     });
     node_tail<DIM, S, R, SHIFT, MACRO>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
                                        load_heat_shift<DIM, SHIFT>(a, idx));
+}
+
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, int WPE = 0>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_step(StepArgs<R, S> a) {
+    int tx, ty, tz;
+    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tz);
+    const int i = tx * blockDim.x + threadIdx.x;
+    const int j = ty * blockDim.y + threadIdx.y;
+    const int k = DIM == 3 ? tz : 0;
+    if (i >= a.n || j >= a.n) return;
+    step_node<DIM, S, R, SHIFT, MACRO>(a, i, j, k);
 }
 
 // Initial collision (iteration 1's collision_U/collision_T): equilibrium

@@ -72,6 +72,26 @@ hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchSha
     return hipGetLastError();
 }
 
+template <bool MACRO>
+__global__ __launch_bounds__(1024) void k_step_batch(const StepArgs<float> *__restrict__ args,
+                                                     const int4 *__restrict__ tiles, int parity) {
+    const int4 t = tiles[blockIdx.x];
+    const StepArgs<float> a = args[t.w ^ parity];
+    const int i = t.y * (int)blockDim.x + (int)threadIdx.x;
+    const int j = t.z * (int)blockDim.y + (int)threadIdx.y;
+    if (i >= a.n || j >= a.n) return;
+    step_node<2, float, float, true, MACRO>(a, i, j, 0);
+}
+
+hipError_t launch_step_batch_f32(bool macro, const StepArgs<float> *args, const int4 *tiles,
+                                 int ntiles, int parity, int bx, int by, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    dim3 block(bx, by, 1), grid(ntiles);
+    if (macro) hipLaunchKernelGGL(k_step_batch<true>, grid, block, 0, s, args, tiles, parity);
+    else hipLaunchKernelGGL(k_step_batch<false>, grid, block, 0, s, args, tiles, parity);
+    return hipGetLastError();
+}
+
 hipError_t launch_prologue_f32(int dim, const StepArgs<float> &a,
                                const StepArgs<double> &ad, hipStream_t s) {
     long nn = dim == 2 ? (long)a.n * a.n : (long)a.n * a.n * a.n;

@@ -37,6 +37,11 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_get_grid", "lbm_get_info",
            "lbm_time_steps", "lbm_synchronize", "lbm_state_size",
            "lbm_save_state", "lbm_load_state", "lbm_set_launch",
            "lbm_get_launch", "lbm_autotune", "lbm_bytes_per_update",
+           "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_cavity",
+           "lbm_batch_init_rest", "lbm_batch_step", "lbm_batch_check",
+           "lbm_batch_nusselt", "lbm_batch_run", "lbm_batch_active",
+           "lbm_batch_time_steps", "lbm_batch_set_launch",
+           "lbm_batch_get_launch", "lbm_batch_autotune",
            "lbm_status_string", "lbm_last_error"]
 
 
@@ -99,11 +104,25 @@ def load():
             "lbm_set_launch": [H, ctypes.POINTER(Launch)],
             "lbm_get_launch": [H, ctypes.POINTER(Launch)],
             "lbm_autotune": [H, I32, DP],
+            "lbm_batch_create": [ctypes.POINTER(Params), I32, ctypes.POINTER(H)],
+            "lbm_batch_destroy": [H],
+            "lbm_batch_init_rest": [H],
+            "lbm_batch_step": [H, I64],
+            "lbm_batch_check": [H, DP, DP],
+            "lbm_batch_nusselt": [H, DP, DP],
+            "lbm_batch_run": [H, ctypes.POINTER(I64), DP, DP, ctypes.POINTER(I32)],
+            "lbm_batch_active": [H, ctypes.POINTER(I32), ctypes.POINTER(I64)],
+            "lbm_batch_time_steps": [H, I64, DP],
+            "lbm_batch_set_launch": [H, ctypes.POINTER(Launch)],
+            "lbm_batch_get_launch": [H, ctypes.POINTER(Launch)],
+            "lbm_batch_autotune": [H, I32, DP],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = I32
+        L.lbm_batch_cavity.argtypes = [H, I32]
+        L.lbm_batch_cavity.restype = H
         L.lbm_status_string.argtypes = [I32]
         L.lbm_status_string.restype = ctypes.c_char_p
         L.lbm_last_error.argtypes = []
@@ -141,11 +160,23 @@ class Solver:
         _check(self._L.lbm_create(ctypes.byref(p), ctypes.byref(self.h)))
         self.params = p
         self.dim, self.n = dim, n
+        self._owned = True
+
+    @classmethod
+    def _borrow(cls, handle, params):
+        """A view of a cavity owned by a Batch (never destroyed by the view)."""
+        s = cls.__new__(cls)
+        s._L = load()
+        s.h = ctypes.c_void_p(handle)
+        s.params = params
+        s.dim, s.n = params.dim, params.nx
+        s._owned = False
+        return s
 
     def close(self):
-        if self.h:
+        if self.h and self._owned:
             self._L.lbm_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -257,3 +288,103 @@ class Solver:
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
         _check(self._L.lbm_load_state(self.h, buf.ctypes.data_as(ctypes.c_void_p),
                                       buf.size))
+
+
+class Batch:
+    """Independent 2D cavities of any sizes and parameters, advanced
+    together: one kernel launch per time step covers every active cavity.
+    `cavities` is a list of dicts of Solver keyword arguments (n, Ra, Pr,
+    stretch, epsU, epsT, itc_max, ...).  batch[i] is a Solver view of
+    cavity i for the read-only calls and per-cavity initialisation."""
+
+    def __init__(self, cavities, device=0, check_every=2000):
+        self._L = load()
+        self.count = len(cavities)
+        arr = (Params * self.count)()
+        for i, c in enumerate(cavities):
+            c = dict(c)
+            n = c.pop("n")
+            arr[i] = Params(dim=2, nx=n, ny=n, nz=1, Ra=c.pop("Ra"), Pr=c.pop("Pr", 0.71),
+                            Ma=c.pop("Ma", 0.1), Thot=c.pop("Thot", 1.0),
+                            Tcold=c.pop("Tcold", 0.0), stretch=c.pop("stretch", 3.2),
+                            grid_kind=c.pop("grid_kind", GRID_ERF),
+                            precision=PREC_FP32_SHIFTED, epsU=c.pop("epsU", 1e-6),
+                            epsT=c.pop("epsT", 1e-6), check_every=check_every,
+                            device=device, itc_max=c.pop("itc_max", 20000))
+            if c:
+                raise TypeError(f"unknown cavity parameters {sorted(c)}")
+        self.params = arr
+        self.h = ctypes.c_void_p()
+        _check(self._L.lbm_batch_create(arr, self.count, ctypes.byref(self.h)))
+        self.cavities = [Solver._borrow(self._L.lbm_batch_cavity(self.h, i), arr[i])
+                         for i in range(self.count)]
+
+    def close(self):
+        if self.h:
+            for c in self.cavities:
+                c.h = ctypes.c_void_p()
+            self._L.lbm_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return self.count
+
+    def __getitem__(self, i):
+        return self.cavities[i]
+
+    def init_rest(self):
+        _check(self._L.lbm_batch_init_rest(self.h))
+
+    def step(self, nsteps=1):
+        _check(self._L.lbm_batch_step(self.h, int(nsteps)))
+
+    def check(self):
+        eu, et = np.zeros(self.count), np.zeros(self.count)
+        _check(self._L.lbm_batch_check(self.h, _dp(eu), _dp(et)))
+        return eu, et
+
+    def nusselt(self):
+        a, b = np.zeros(self.count), np.zeros(self.count)
+        _check(self._L.lbm_batch_nusselt(self.h, _dp(a), _dp(b)))
+        return a, b
+
+    def run(self):
+        """Each cavity to its own eps / itc_max; returns (itc, errU, errT,
+        status) arrays."""
+        itc = np.zeros(self.count, dtype=np.int64)
+        eu, et = np.zeros(self.count), np.zeros(self.count)
+        st = np.zeros(self.count, dtype=np.int32)
+        _check(self._L.lbm_batch_run(
+            self.h, itc.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _dp(eu), _dp(et),
+            st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return itc, eu, et, st
+
+    def active(self):
+        k, m = ctypes.c_int32(), ctypes.c_int64()
+        _check(self._L.lbm_batch_active(self.h, ctypes.byref(k), ctypes.byref(m)))
+        return k.value, m.value
+
+    def time_steps(self, nsteps):
+        ms = ctypes.c_double()
+        _check(self._L.lbm_batch_time_steps(self.h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def set_launch(self, bx, by):
+        cfg = Launch(bx=bx, by=by, remap=0, occupancy=0, variant=0)
+        _check(self._L.lbm_batch_set_launch(self.h, ctypes.byref(cfg)))
+
+    def get_launch(self):
+        cfg = Launch()
+        _check(self._L.lbm_batch_get_launch(self.h, ctypes.byref(cfg)))
+        return {k: getattr(cfg, k) for k, _ in Launch._fields_}
+
+    def autotune(self, steps_per_trial=10):
+        ms = ctypes.c_double()
+        _check(self._L.lbm_batch_autotune(self.h, int(steps_per_trial), ctypes.byref(ms)))
+        return ms.value

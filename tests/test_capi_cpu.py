@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "lbm_capi.h")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int32_t|const char \*)\s*(lbm_\w+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:int32_t|const char \*|lbm_handle)\s*(lbm_\w+)\s*\(",
                                  text, flags=re.M)))
 
 
@@ -51,3 +51,13 @@ def test_status_strings(hip):
     lib = hip.load()
     for code in range(7):
         assert lib.lbm_status_string(code)
+
+
+def test_batch_rejects_bad_parameters(hip):
+    ok = dict(n=17, Ra=1e4)
+    with pytest.raises(hip.LbmError):
+        hip.Batch([])                                    # no cavity
+    with pytest.raises(hip.LbmError):
+        hip.Batch([ok, dict(n=3, Ra=1e4)])               # cavity 1 invalid
+    with pytest.raises(TypeError):
+        hip.Batch([dict(n=17, Ra=1e4, colour="red")])

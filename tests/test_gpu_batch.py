@@ -1,0 +1,134 @@
+"""Batches of independent 2D cavities (lbm_batch_*): one launch per time step
+advances every active cavity.  Each cavity must follow the single-cavity
+semantics (natural-convection_2dim_mono.f90 per cavity): the batch is
+compared with independent single-cavity solvers on the same parameters and
+with the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from helpers import field_errors
+
+pytestmark = pytest.mark.gpu
+
+# uneven sizes, stretch factors, Rayleigh and Prandtl numbers
+CAVITIES = [dict(n=17, Ra=1e3, Pr=0.71, stretch=1.0),
+            dict(n=33, Ra=1e5, Pr=7.0, stretch=2.5),
+            dict(n=24, Ra=1e6, Pr=0.7, stretch=3.2),
+            dict(n=65, Ra=3e4, Pr=2.0, stretch=1.7),
+            dict(n=40, Ra=1e4, Pr=0.71, stretch=3.2)]
+
+
+def _single(hip, c, **kw):
+    c = dict(c)
+    n = c.pop("n")
+    return hip.Solver(2, n, c.pop("Ra"), **c, **kw)
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def test_batch_matches_single_cavities(hip):
+    print("\n[batch] 5 cavities vs single solvers, 300 steps", flush=True)
+    b = hip.Batch(CAVITIES)
+    b.set_launch(64, 4)
+    b.init_rest()
+    b.step(300)
+    for i, c in enumerate(CAVITIES):
+        s = _single(hip, c)
+        s.set_launch(64, 4, remap=0)
+        s.init_rest()
+        s.step(300)
+        got, ref = b[i].fields(), s.fields()
+        assert b[i].itc == s.itc == 300
+        errs = [_rel(x, y) for x, y in zip(got, ref) if np.abs(y).max() > 0]
+        print(f"  cavity {i} n={c['n']}: max rel diff {max(errs):.3g}", flush=True)
+        # the same node code in another kernel: FP32 rounding at most
+        assert max(errs) <= 1e-6, (i, errs)
+        nb, ns = b[i].nusselt(), s.nusselt()
+        assert abs(nb[0] - ns[0]) <= 1e-6 * abs(ns[0])
+        s.close()
+    b.close()
+
+
+def test_batch_matches_oracle(hip, oracle_mod):
+    print("\n[batch] vs oracle, 2 cavities, 1000 steps", flush=True)
+    cav = [CAVITIES[0], CAVITIES[1]]
+    b = hip.Batch(cav)
+    b.init_rest()
+    b.step(1000)
+    for i, c in enumerate(cav):
+        o = oracle_mod.Oracle(2, c["n"], c["Ra"], Pr=c["Pr"], A=c["stretch"])
+        o.init_rest()
+        o.step(1000)
+        errs = field_errors(b[i].fields(), o.fields())
+        print(f"  cavity {i}: {errs}", flush=True)
+        for k in ("rho", "u", "v", "T"):
+            assert errs[k] <= 1e-5, (i, k, errs[k])
+        nh, nc = b[i].nusselt()
+        oh, oc = o.nusselt()
+        assert abs(nh - oh) <= 1e-4 * abs(oh) and abs(nc - oc) <= 1e-4 * abs(oc)
+        o.close()
+    b.close()
+
+
+def test_batch_run_stops_each_cavity_on_its_own_rule(hip):
+    print("\n[batch] run with per-cavity eps and itc_max", flush=True)
+    cav = [dict(n=17, Ra=1e3, epsU=1e-2, epsT=1e-2, itc_max=40000),
+           dict(n=24, Ra=1e4, epsU=1e-9, epsT=1e-9, itc_max=6000),     # stops at itc_max
+           dict(n=20, Ra=1e4, stretch=1.5, epsU=1e-3, epsT=1e-3, itc_max=40000)]
+    b = hip.Batch(cav, check_every=1000)
+    b.init_rest()
+    itc, eu, et, st = b.run()
+    print(f"  itc={itc.tolist()} status={st.tolist()} errU={eu.tolist()}", flush=True)
+    assert st.tolist() == [hip.LBM_OK, hip.LBM_ERR_NOT_CONVERGED, hip.LBM_OK]
+    assert itc[1] == 6000
+    assert b.active()[0] == 0
+    for i, c in enumerate(cav):
+        s = _single(hip, c, check_every=1000)
+        s.set_launch(*[b.get_launch()[k] for k in ("bx", "by")], remap=0)
+        s.init_rest()
+        code, sitc, seu, set_ = s.run()
+        assert code == st[i] and sitc == itc[i], (i, code, sitc)
+        assert abs(seu - eu[i]) <= 1e-5 * seu and abs(set_ - et[i]) <= 1e-5 * set_
+        s.close()
+    b.close()
+
+
+def test_batch_warm_start_of_one_cavity(hip):
+    print("\n[batch] init_fields on one cavity", flush=True)
+    b = hip.Batch(CAVITIES[:3])
+    b.init_rest()
+    n = CAVITIES[1]["n"]
+    rng = np.random.default_rng(5)
+    u = 1e-3 * rng.standard_normal((n, n))
+    T = 0.5 + 0.1 * rng.standard_normal((n, n))
+    b[1].init_fields(u, None, None, T)
+    b.step(50)
+    s = _single(hip, CAVITIES[1])
+    s.set_launch(64, 4, remap=0)
+    s.init_fields(u, None, None, T)
+    s.step(50)
+    errs = [_rel(x, y) for x, y in zip(b[1].fields(), s.fields()) if np.abs(y).max() > 0]
+    assert max(errs) <= 1e-6, errs
+    assert b[0].itc == 50 and b[1].itc == 50
+    s.close()
+    b.close()
+
+
+def test_batch_autotune_restores_state(hip):
+    print("\n[batch] autotune", flush=True)
+    b = hip.Batch(CAVITIES)
+    b.init_rest()
+    b.step(20)
+    before = [b[i].fields() for i in range(len(CAVITIES))]
+    ms = b.autotune(3)
+    assert ms > 0
+    for i in range(len(CAVITIES)):
+        assert b[i].itc == 20
+        for x, y in zip(before[i], b[i].fields()):
+            assert np.array_equal(x, y)
+    cav, nodes = b.active()
+    assert cav == len(CAVITIES) and nodes == sum(c["n"] ** 2 for c in CAVITIES)
+    b.close()
