@@ -140,6 +140,76 @@ module lbm_capi
             type(c_ptr), value :: h
             real(c_double), intent(out) :: bytes
         end function
+        ! batches of 2D cavities: arrays hold one value per cavity
+        integer(c_int32_t) function lbm_batch_create(params, count, b) bind(C, name='lbm_batch_create')
+            import :: c_int32_t, c_ptr, lbm_params
+            type(lbm_params), intent(in) :: params(*)
+            integer(c_int32_t), value :: count
+            type(c_ptr), intent(out) :: b
+        end function
+        integer(c_int32_t) function lbm_batch_destroy(b) bind(C, name='lbm_batch_destroy')
+            import :: c_int32_t, c_ptr
+            type(c_ptr), value :: b
+        end function
+        type(c_ptr) function lbm_batch_cavity(b, index) bind(C, name='lbm_batch_cavity')
+            import :: c_ptr, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: index
+        end function
+        integer(c_int32_t) function lbm_batch_init_rest(b) bind(C, name='lbm_batch_init_rest')
+            import :: c_int32_t, c_ptr
+            type(c_ptr), value :: b
+        end function
+        integer(c_int32_t) function lbm_batch_step(b, nsteps) bind(C, name='lbm_batch_step')
+            import :: c_int32_t, c_ptr, c_int64_t
+            type(c_ptr), value :: b
+            integer(c_int64_t), value :: nsteps
+        end function
+        integer(c_int32_t) function lbm_batch_check(b, errU, errT) bind(C, name='lbm_batch_check')
+            import :: c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: b
+            real(c_double), intent(out) :: errU(*), errT(*)
+        end function
+        integer(c_int32_t) function lbm_batch_nusselt(b, nu_hot, nu_cold) bind(C, name='lbm_batch_nusselt')
+            import :: c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: b
+            real(c_double), intent(out) :: nu_hot(*), nu_cold(*)
+        end function
+        integer(c_int32_t) function lbm_batch_run(b, itc, errU, errT, status) bind(C, name='lbm_batch_run')
+            import :: c_int32_t, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: b
+            integer(c_int64_t), intent(out) :: itc(*)
+            real(c_double), intent(out) :: errU(*), errT(*)
+            integer(c_int32_t), intent(out) :: status(*)
+        end function
+        integer(c_int32_t) function lbm_batch_active(b, cavities, nodes) bind(C, name='lbm_batch_active')
+            import :: c_int32_t, c_ptr, c_int64_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), intent(out) :: cavities
+            integer(c_int64_t), intent(out) :: nodes
+        end function
+        integer(c_int32_t) function lbm_batch_time_steps(b, nsteps, ms) bind(C, name='lbm_batch_time_steps')
+            import :: c_int32_t, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: b
+            integer(c_int64_t), value :: nsteps
+            real(c_double), intent(out) :: ms
+        end function
+        integer(c_int32_t) function lbm_batch_set_launch(b, launch) bind(C, name='lbm_batch_set_launch')
+            import :: c_int32_t, c_ptr, lbm_launch
+            type(c_ptr), value :: b
+            type(lbm_launch), intent(in) :: launch
+        end function
+        integer(c_int32_t) function lbm_batch_get_launch(b, launch) bind(C, name='lbm_batch_get_launch')
+            import :: c_int32_t, c_ptr, lbm_launch
+            type(c_ptr), value :: b
+            type(lbm_launch), intent(out) :: launch
+        end function
+        integer(c_int32_t) function lbm_batch_autotune(b, steps_per_trial, best_ms) bind(C, name='lbm_batch_autotune')
+            import :: c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: steps_per_trial
+            real(c_double), intent(out) :: best_ms
+        end function
         type(c_ptr) function lbm_status_string(status) bind(C, name='lbm_status_string')
             import :: c_ptr, c_int32_t
             integer(c_int32_t), value :: status
