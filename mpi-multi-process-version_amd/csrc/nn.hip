@@ -1,0 +1,625 @@
+// nn.hip -- the learned warm start (include/lbm_nn.h): kernels for gfx950
+// and the host side of the encoder-decoder.
+//
+// Every convolution is an im2col product on the matrix cores:
+//   forward   Z[cout][N]  = W[cout][K] . col[K][N]            (K = cin k^2)
+//   backward  dW[cout][K] = dZ[cout][N] . col[K][N]^T         (split over N)
+//             dcol[K][N]  = W[cout][K]^T . dZ[cout][N], then col2im
+// with N = batch x output pixels.  The product kernel multiplies BF16
+// operands with v_mfma_f32_16x16x16_bf16 and accumulates in FP32; one wave
+// computes one 16 x 16 tile of the result.  Activations are FP32 in
+// channel-major layout [C][B][H][W], so a channel concatenation is a
+// contiguous stack and a layer's product output is its activation.  All
+// reductions run in a fixed order, so every call is deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_nn.h"
+#include "lbm_internal.h"
+
+namespace nn {
+
+using bf16 = uint16_t;
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// round to nearest even; NaN stays NaN
+__device__ __forceinline__ bf16 to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (bf16)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+
+// C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m as_m + k as_k] and
+// B(k, n) = B[k bs_k + n bs_n]; C row-major with leading dimension ldc.
+// blockIdx.z selects the k range [z kchunk, (z + 1) kchunk); a split
+// product writes its partial result to C + z M ldc.
+struct GemmArgs {
+    const bf16 *A;
+    long as_m, as_k;
+    const bf16 *B;
+    long bs_k, bs_n;
+    float *C;
+    long ldc;
+    int M, N, K, kchunk;
+};
+
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int tn = (g.N + 15) >> 4, tm = (g.M + 15) >> 4;
+    if (wave >= tn * tm) return;   // whole waves only
+    const int m0 = (wave / tn) << 4, n0 = (wave % tn) << 4;
+    const int kb = blockIdx.z * g.kchunk, ke = min(g.K, kb + g.kchunk);
+    const int r = lane & 15, q = lane >> 4;
+    const long m = m0 + r, n = n0 + r;
+    const bool mok = m < g.M, nok = n < g.N;
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        v4s a, b;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const long k = k0 + 4 * q + e;
+            const bool kin = k < ke;
+            a[e] = (kin && mok) ? (short)g.A[m * g.as_m + k * g.as_k] : (short)0;
+            b[e] = (kin && nok) ? (short)g.B[k * g.bs_k + n * g.bs_n] : (short)0;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    float *C = g.C + (long)blockIdx.z * g.M * g.ldc;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const long mi = m0 + 4 * q + e, nj = n0 + r;
+        if (mi < g.M && nj < g.N) C[mi * g.ldc + nj] = acc[e];
+    }
+}
+
+// C[m ldc + n] = sum_z P[z M N + m N + n], z in order
+__global__ void k_reduce_splits(const float *P, float *C, long ldc, int M, int N, int splits) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)M * N) return;
+    const long m = t / N, n = t % N;
+    float s = 0.0f;
+    for (int z = 0; z < splits; z++) s += P[(long)z * M * N + t];
+    C[m * ldc + n] = s;
+}
+
+// col[(c k^2 + ky k + kx) N + n] = X[c][b][yo s + ky - p][xo s + kx - p], n = (b Ho + yo) Wo + xo
+__global__ void k_im2col(const float *X, bf16 *col, int C, int B, int H, int W, int ks,
+                         int stride, int pad, int Ho, int Wo) {
+    const long N = (long)B * Ho * Wo;
+    const long total = (long)C * ks * ks * N;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const long n = t % N, kk = t / N;
+        const int c = (int)(kk / (ks * ks)), rr = (int)(kk % (ks * ks));
+        const int ky = rr / ks, kx = rr % ks;
+        const int xo = (int)(n % Wo);
+        const long t2 = n / Wo;
+        const int yo = (int)(t2 % Ho), b = (int)(t2 / Ho);
+        const int y = yo * stride + ky - pad, x = xo * stride + kx - pad;
+        float v = 0.0f;
+        if (y >= 0 && y < H && x >= 0 && x < W) v = X[(((long)c * B + b) * H + y) * W + x];
+        col[t] = to_bf16(v);
+    }
+}
+
+// dX[c][b][y][x] (+)= sum of the dcol entries that im2col read from (c, b, y, x)
+__global__ void k_col2im(const float *dcol, float *dX, int C, int B, int H, int W, int ks,
+                         int stride, int pad, int Ho, int Wo, int accumulate) {
+    const long N = (long)B * Ho * Wo;
+    const long total = (long)C * B * H * W;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % W);
+        const long t2 = t / W;
+        const int y = (int)(t2 % H);
+        const long t3 = t2 / H;
+        const int b = (int)(t3 % B), c = (int)(t3 / B);
+        float s = 0.0f;
+        for (int ky = 0; ky < ks; ky++) {
+            const int yy = y + pad - ky;
+            if (yy < 0 || yy % stride) continue;
+            const int yo = yy / stride;
+            if (yo >= Ho) continue;
+            for (int kx = 0; kx < ks; kx++) {
+                const int xx = x + pad - kx;
+                if (xx < 0 || xx % stride) continue;
+                const int xo = xx / stride;
+                if (xo >= Wo) continue;
+                s += dcol[((long)c * ks * ks + ky * ks + kx) * N + ((long)b * Ho + yo) * Wo + xo];
+            }
+        }
+        dX[t] = accumulate ? dX[t] + s : s;
+    }
+}
+
+__global__ void k_bias_act(float *Y, const float *bias, int rows, long N, int relu) {
+    const long total = (long)rows * N;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        float v = Y[t] + bias[t / N];
+        Y[t] = relu ? fmaxf(v, 0.0f) : v;
+    }
+}
+
+// dZ = dA where the ReLU output A is positive, else 0 (in place)
+__global__ void k_relu_bwd(float *dA, const float *A, long n) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x)
+        if (!(A[t] > 0.0f)) dA[t] = 0.0f;
+}
+
+__global__ void k_to_bf16(const float *x, bf16 *y, long n) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x)
+        y[t] = to_bf16(x[t]);
+}
+
+// out[r] = sum_n X[r N + n]: one block per row, fixed order
+__global__ __launch_bounds__(256) void k_rowsum(const float *X, float *out, long N) {
+    __shared__ float sh[256];
+    const float *row = X + (long)blockIdx.x * N;
+    float s = 0.0f;
+    for (long n = threadIdx.x; n < N; n += 256) s += row[n];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+// nearest x2 upsampling of [C][B][h][w] and its adjoint
+__global__ void k_up2(const float *X, float *Y, long CB, int h, int w) {
+    const long total = CB * 4 * h * w;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % (2 * w));
+        const long t2 = t / (2 * w);
+        const int y = (int)(t2 % (2 * h));
+        const long cb = t2 / (2 * h);
+        Y[t] = X[(cb * h + y / 2) * w + x / 2];
+    }
+}
+__global__ void k_up2_bwd(const float *dY, float *dX, long CB, int h, int w) {
+    const long total = CB * h * w;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % w);
+        const long t2 = t / w;
+        const int y = (int)(t2 % h);
+        const long cb = t2 / h;
+        const float *d = dY + (cb * 2 * h + 2 * y) * (2 * w) + 2 * x;
+        dX[t] = (d[0] + d[1]) + (d[2 * w] + d[2 * w + 1]);
+    }
+}
+
+// sample-major [B][C][P] <-> channel-major [C][B][P]
+__global__ void k_to_cmajor(const float *src, float *dst, int B, int C, long P) {
+    const long total = (long)B * C * P;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const long p = t % P, t2 = t / P;
+        const int c = (int)(t2 % C), b = (int)(t2 / C);
+        dst[((long)c * B + b) * P + p] = src[t];
+    }
+}
+__global__ void k_to_smajor(const float *src, float *dst, int B, int C, long P) {
+    const long total = (long)B * C * P;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const long p = t % P, t2 = t / P;
+        const int c = (int)(t2 % C), b = (int)(t2 / C);
+        dst[t] = src[((long)c * B + b) * P + p];
+    }
+}
+
+// dY = 2 (Y - T) / n; part[block] = sum over the block of (Y - T)^2
+__global__ __launch_bounds__(256) void k_mse(const float *Y, const float *T, float *dY,
+                                             float *part, long n) {
+    __shared__ float sh[256];
+    float s = 0.0f;
+    const float scale = 2.0f / (float)n;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const float d = Y[t] - T[t];
+        dY[t] = scale * d;
+        s += d * d;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_adam(float *w, const float *g, float *m, float *v, long n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const float gi = g[t];
+        const float mi = b1 * m[t] + (1.0f - b1) * gi;
+        const float vi = b2 * v[t] + (1.0f - b2) * gi * gi;
+        m[t] = mi;
+        v[t] = vi;
+        w[t] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+constexpr int kMseBlocks = 256;
+
+unsigned grid_for(long n) { return (unsigned)std::min<long>((n + 255) / 256, 65535L); }
+
+}  // namespace nn
+
+using nn::bf16;
+
+namespace {
+
+struct ConvLayer {
+    int cin, cout, ks, stride, relu;
+    long woff, boff;   // offsets into the flat parameter array
+};
+
+}  // namespace
+
+struct lbm_nn_ctx {
+    int G = 0, Bmax = 0, device = 0;
+    hipStream_t s = nullptr;
+    std::vector<ConvLayer> L;
+    long np = 0;
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    bf16 *wb = nullptr;
+    long t = 0;   // Adam steps taken
+    // activations and their gradients, [C][B][H][W]
+    float *x0 = nullptr, *c7 = nullptr, *c6 = nullptr, *c5 = nullptr, *a4 = nullptr,
+          *a5 = nullptr, *a6 = nullptr, *a7 = nullptr, *y = nullptr;
+    float *dc7 = nullptr, *dc6 = nullptr, *dc5 = nullptr, *da4 = nullptr, *da5 = nullptr,
+          *da6 = nullptr, *da7 = nullptr, *dy = nullptr;
+    bf16 *col = nullptr, *dzb = nullptr;
+    float *dcol = nullptr, *part = nullptr, *red = nullptr, *stage = nullptr, *tgt = nullptr;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+int32_t fail_hip(hipError_t e, const char *where) {
+    return lbm::set_error(LBM_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define NN_TRY(expr)                                         \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return fail_hip(e_, #expr);    \
+    } while (0)
+
+void release(lbm_nn n) {
+    for (void *p : n->allocs) (void)hipFree(p);
+    if (n->s) (void)hipStreamDestroy(n->s);
+    delete n;
+}
+
+template <typename T> int32_t dalloc(lbm_nn n, T *&p, long count) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, sizeof(T) * (size_t)std::max(count, 1L));
+    if (e != hipSuccess)
+        return lbm::set_error(LBM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    n->allocs.push_back(q);
+    p = (T *)q;
+    return LBM_OK;
+}
+
+// splitmix64: the weight initialisation's random stream
+uint64_t splitmix(uint64_t &s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+hipError_t gemm(lbm_nn n, int M, int N, int K, const bf16 *A, long as_m, long as_k,
+                const bf16 *B, long bs_k, long bs_n, float *C, long ldc) {
+    // split the k range when the output is small and k is long
+    const long tiles = (long)((M + 15) / 16) * ((N + 15) / 16);
+    int splits = 1;
+    if (tiles < 1024 && K >= 4096) splits = (int)std::min<long>(64, std::max<long>(1, K / 2048));
+    const int kchunk = ((K + splits - 1) / splits + 15) / 16 * 16;
+    splits = (K + kchunk - 1) / kchunk;
+    nn::GemmArgs g{A, as_m, as_k, B, bs_k, bs_n, splits > 1 ? n->part : C,
+                   splits > 1 ? (long)N : ldc, M, N, K, kchunk};
+    dim3 grid((unsigned)((tiles + 3) / 4), 1, (unsigned)splits);
+    hipLaunchKernelGGL(nn::k_gemm, grid, dim3(256), 0, n->s, g);
+    if (splits > 1) {
+        const long mn = (long)M * N;
+        hipLaunchKernelGGL(nn::k_reduce_splits, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0,
+                           n->s, (const float *)n->part, C, ldc, M, N, splits);
+    }
+    return hipGetLastError();
+}
+
+// forward of one convolution: Y[cout][B][Ho][Wo] = act(W * X + b)
+hipError_t conv_fwd(lbm_nn n, const ConvLayer &l, int B, const float *X, int H, float *Y) {
+    const int Ho = H / l.stride, pad = l.ks / 2;
+    const long N = (long)B * Ho * Ho, K = (long)l.cin * l.ks * l.ks;
+    hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(K * N)), dim3(256), 0, n->s, X, n->col,
+                       l.cin, B, H, H, l.ks, l.stride, pad, Ho, Ho);
+    hipError_t e = gemm(n, l.cout, (int)N, (int)K, n->wb + l.woff, K, 1, n->col, N, 1, Y, N);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(nn::k_bias_act, dim3(nn::grid_for((long)l.cout * N)), dim3(256), 0, n->s,
+                       Y, (const float *)(n->w + l.boff), l.cout, N, l.relu);
+    return hipGetLastError();
+}
+
+// backward of one convolution.  dA (the gradient of its output) is turned
+// into dZ in place; dW, db go to the gradient array; dX is written or added
+// to when given.
+hipError_t conv_bwd(lbm_nn n, const ConvLayer &l, int B, const float *X, int H, const float *A,
+                    float *dA, float *dX, bool accumulate) {
+    const int Ho = H / l.stride, pad = l.ks / 2;
+    const long N = (long)B * Ho * Ho, K = (long)l.cin * l.ks * l.ks;
+    if (l.relu)
+        hipLaunchKernelGGL(nn::k_relu_bwd, dim3(nn::grid_for(l.cout * N)), dim3(256), 0, n->s, dA,
+                           A, (long)l.cout * N);
+    hipLaunchKernelGGL(nn::k_rowsum, dim3(l.cout), dim3(256), 0, n->s, (const float *)dA,
+                       n->g + l.boff, N);
+    hipLaunchKernelGGL(nn::k_to_bf16, dim3(nn::grid_for(l.cout * N)), dim3(256), 0, n->s,
+                       (const float *)dA, n->dzb, (long)l.cout * N);
+    hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(K * N)), dim3(256), 0, n->s, X, n->col,
+                       l.cin, B, H, H, l.ks, l.stride, pad, Ho, Ho);
+    // dW[cout][K] = dZ[cout][N] . col[K][N]^T
+    hipError_t e = gemm(n, l.cout, (int)K, (int)N, n->dzb, N, 1, n->col, 1, N, n->g + l.woff, K);
+    if (e != hipSuccess || !dX) return e;
+    // dcol[K][N] = W^T . dZ
+    e = gemm(n, (int)K, (int)N, l.cout, n->wb + l.woff, 1, K, n->dzb, N, 1, n->dcol, N);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(nn::k_col2im, dim3(nn::grid_for((long)l.cin * B * H * H)), dim3(256), 0,
+                       n->s, (const float *)n->dcol, dX, l.cin, B, H, H, l.ks, l.stride, pad, Ho,
+                       Ho, accumulate ? 1 : 0);
+    return hipGetLastError();
+}
+
+int32_t check_net(lbm_nn n, int32_t batch) {
+    if (!n) return lbm::set_error(LBM_ERR_ARG, "null network");
+    if (batch < 1 || batch > n->Bmax) return lbm::set_error(LBM_ERR_ARG, "batch out of range");
+    NN_TRY(hipSetDevice(n->device));
+    return LBM_OK;
+}
+
+// x (host, sample-major) -> y (device, channel-major) through the network
+int32_t forward(lbm_nn n, int B, const float *x_host) {
+    const int G = n->G;
+    const long P = (long)G * G;
+    NN_TRY(hipMemcpyAsync(n->stage, x_host, sizeof(float) * B * LBM_NN_IN_CHANNELS * P,
+                          hipMemcpyHostToDevice, n->s));
+    hipLaunchKernelGGL(nn::k_to_cmajor, dim3(nn::grid_for(B * 4 * P)), dim3(256), 0, n->s,
+                       (const float *)n->stage, n->x0, B, 4, P);
+    hipLaunchKernelGGL(nn::k_to_bf16, dim3(nn::grid_for(n->np)), dim3(256), 0, n->s,
+                       (const float *)n->w, n->wb, n->np);
+    const int G2 = G / 2, G4 = G / 4, G8 = G / 8;
+    float *A1 = n->c7 + 32L * B * P, *A2 = n->c6 + 64L * B * G2 * G2,
+          *A3 = n->c5 + 64L * B * G4 * G4;
+    hipError_t e = hipSuccess;
+    auto run = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    run(conv_fwd(n, n->L[0], B, n->x0, G, A1));
+    run(conv_fwd(n, n->L[1], B, A1, G, A2));
+    run(conv_fwd(n, n->L[2], B, A2, G2, A3));
+    run(conv_fwd(n, n->L[3], B, A3, G4, n->a4));
+    hipLaunchKernelGGL(nn::k_up2, dim3(nn::grid_for(64L * B * G4 * G4)), dim3(256), 0, n->s,
+                       (const float *)n->a4, n->c5, 64L * B, G8, G8);
+    run(conv_fwd(n, n->L[4], B, n->c5, G4, n->a5));
+    hipLaunchKernelGGL(nn::k_up2, dim3(nn::grid_for(64L * B * G2 * G2)), dim3(256), 0, n->s,
+                       (const float *)n->a5, n->c6, 64L * B, G4, G4);
+    run(conv_fwd(n, n->L[5], B, n->c6, G2, n->a6));
+    hipLaunchKernelGGL(nn::k_up2, dim3(nn::grid_for(32L * B * P)), dim3(256), 0, n->s,
+                       (const float *)n->a6, n->c7, 32L * B, G2, G2);
+    run(conv_fwd(n, n->L[6], B, n->c7, G, n->a7));
+    run(conv_fwd(n, n->L[7], B, n->a7, G, n->y));
+    run(hipGetLastError());
+    if (e != hipSuccess) return fail_hip(e, "forward");
+    return LBM_OK;
+}
+
+// loss of y against target (host, sample-major); fills dy and the gradient
+int32_t backward(lbm_nn n, int B, const float *target, float *loss) {
+    const int G = n->G, G2 = G / 2, G4 = G / 4, G8 = G / 8;
+    const long P = (long)G * G;
+    const long ny = 3L * B * P;
+    NN_TRY(hipMemcpyAsync(n->stage, target, sizeof(float) * ny, hipMemcpyHostToDevice, n->s));
+    hipLaunchKernelGGL(nn::k_to_cmajor, dim3(nn::grid_for(ny)), dim3(256), 0, n->s,
+                       (const float *)n->stage, n->tgt, B, 3, P);
+    hipLaunchKernelGGL(nn::k_mse, dim3(nn::kMseBlocks), dim3(256), 0, n->s, (const float *)n->y,
+                       (const float *)n->tgt, n->dy, n->red, ny);
+    float *A1 = n->c7 + 32L * B * P, *A2 = n->c6 + 64L * B * G2 * G2,
+          *A3 = n->c5 + 64L * B * G4 * G4;
+    float *dA1 = n->dc7 + 32L * B * P, *dA2 = n->dc6 + 64L * B * G2 * G2,
+          *dA3 = n->dc5 + 64L * B * G4 * G4;
+    hipError_t e = hipSuccess;
+    auto run = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    run(conv_bwd(n, n->L[7], B, n->a7, G, n->y, n->dy, n->da7, false));
+    run(conv_bwd(n, n->L[6], B, n->c7, G, n->a7, n->da7, n->dc7, false));
+    hipLaunchKernelGGL(nn::k_up2_bwd, dim3(nn::grid_for(32L * B * G2 * G2)), dim3(256), 0, n->s,
+                       (const float *)n->dc7, n->da6, 32L * B, G2, G2);
+    run(conv_bwd(n, n->L[5], B, n->c6, G2, n->a6, n->da6, n->dc6, false));
+    hipLaunchKernelGGL(nn::k_up2_bwd, dim3(nn::grid_for(64L * B * G4 * G4)), dim3(256), 0, n->s,
+                       (const float *)n->dc6, n->da5, 64L * B, G4, G4);
+    run(conv_bwd(n, n->L[4], B, n->c5, G4, n->a5, n->da5, n->dc5, false));
+    hipLaunchKernelGGL(nn::k_up2_bwd, dim3(nn::grid_for(64L * B * G8 * G8)), dim3(256), 0, n->s,
+                       (const float *)n->dc5, n->da4, 64L * B, G8, G8);
+    run(conv_bwd(n, n->L[3], B, A3, G4, n->a4, n->da4, dA3, true));
+    run(conv_bwd(n, n->L[2], B, A2, G2, A3, dA3, dA2, true));
+    run(conv_bwd(n, n->L[1], B, A1, G, A2, dA2, dA1, true));
+    run(conv_bwd(n, n->L[0], B, n->x0, G, A1, dA1, nullptr, false));
+    run(hipGetLastError());
+    if (e != hipSuccess) return fail_hip(e, "backward");
+    std::vector<float> part(nn::kMseBlocks);
+    NN_TRY(hipMemcpyAsync(part.data(), n->red, sizeof(float) * part.size(),
+                          hipMemcpyDeviceToHost, n->s));
+    NN_TRY(hipStreamSynchronize(n->s));
+    double s = 0.0;
+    for (float p : part) s += p;
+    if (loss) *loss = (float)(s / (double)ny);
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t lbm_nn_create(int32_t grid, int32_t batch_max, uint64_t seed, int32_t device,
+                      lbm_nn *out) {
+    if (!out) return lbm::set_error(LBM_ERR_ARG, "null output");
+    *out = nullptr;
+    if (grid < 16 || grid % 8 || batch_max < 1 || (long)batch_max * grid * grid > (1L << 26))
+        return lbm::set_error(LBM_ERR_ARG, "grid must be a multiple of 8 (>= 16), batch >= 1");
+    lbm_nn n = new (std::nothrow) lbm_nn_ctx();
+    if (!n) return lbm::set_error(LBM_ERR_NOMEM, "network");
+    n->G = grid;
+    n->Bmax = batch_max;
+    n->device = device;
+    // layers: cin, cout, k, stride, relu
+    const int spec[8][5] = {{4, 16, 3, 1, 1},  {16, 32, 3, 2, 1}, {32, 64, 3, 2, 1},
+                            {64, 64, 3, 2, 1}, {128, 64, 3, 1, 1}, {96, 32, 3, 1, 1},
+                            {48, 16, 3, 1, 1}, {16, 3, 1, 1, 0}};
+    long off = 0;
+    for (auto &sp : spec) {
+        ConvLayer l{sp[0], sp[1], sp[2], sp[3], sp[4], 0, 0};
+        l.woff = off;
+        off += (long)l.cout * l.cin * l.ks * l.ks;
+        l.boff = off;
+        off += l.cout;
+        n->L.push_back(l);
+    }
+    n->np = off;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        release(n);
+        return fail_hip(e, "network stream");
+    }
+    const long B = batch_max, P = (long)grid * grid;
+    const long P2 = P / 4, P4 = P / 16, P8 = P / 64;
+    // largest im2col matrix: K N over the layers
+    long colmax = 0, zmax = 0;
+    {
+        const int hin[8] = {grid, grid, grid / 2, grid / 4, grid / 4, grid / 2, grid, grid};
+        for (int i = 0; i < 8; i++) {
+            const ConvLayer &l = n->L[i];
+            const long Ho = hin[i] / l.stride;
+            const long N = B * Ho * Ho;
+            colmax = std::max(colmax, (long)l.cin * l.ks * l.ks * N);
+            zmax = std::max(zmax, (long)l.cout * N);
+        }
+    }
+    int32_t rc = LBM_OK;
+    auto A = [&](auto *&p, long cnt) { if (!rc) rc = dalloc(n, p, cnt); };
+    A(n->w, n->np); A(n->g, n->np); A(n->m, n->np); A(n->v, n->np); A(n->wb, n->np);
+    A(n->x0, 4 * B * P); A(n->c7, 48 * B * P); A(n->c6, 96 * B * P2); A(n->c5, 128 * B * P4);
+    A(n->a4, 64 * B * P8); A(n->a5, 64 * B * P4); A(n->a6, 32 * B * P2); A(n->a7, 16 * B * P);
+    A(n->y, 3 * B * P);
+    A(n->dc7, 48 * B * P); A(n->dc6, 96 * B * P2); A(n->dc5, 128 * B * P4);
+    A(n->da4, 64 * B * P8); A(n->da5, 64 * B * P4); A(n->da6, 32 * B * P2); A(n->da7, 16 * B * P);
+    A(n->dy, 3 * B * P);
+    A(n->col, colmax); A(n->dzb, zmax); A(n->dcol, colmax);
+    A(n->part, 64L * 64 * 128 * 9); A(n->red, nn::kMseBlocks);
+    A(n->stage, 4 * B * P); A(n->tgt, 3 * B * P);
+    if (rc) {
+        release(n);
+        return rc;
+    }
+    // He-uniform weights, zero biases
+    std::vector<float> w(n->np, 0.0f);
+    uint64_t st = seed;
+    for (const ConvLayer &l : n->L) {
+        const double lim = std::sqrt(6.0 / (double)(l.cin * l.ks * l.ks));
+        for (long i = 0; i < (long)l.cout * l.cin * l.ks * l.ks; i++) {
+            const double u = (double)(splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
+            w[l.woff + i] = (float)((2.0 * u - 1.0) * lim);
+        }
+    }
+    *out = n;
+    return lbm_nn_set_params(n, w.data());
+}
+
+int32_t lbm_nn_destroy(lbm_nn n) {
+    if (!n) return LBM_OK;
+    (void)hipSetDevice(n->device);
+    (void)hipStreamSynchronize(n->s);
+    release(n);
+    return LBM_OK;
+}
+
+int32_t lbm_nn_param_count(lbm_nn n, int64_t *count) {
+    if (!n || !count) return lbm::set_error(LBM_ERR_ARG, "null argument");
+    *count = n->np;
+    return LBM_OK;
+}
+
+int32_t lbm_nn_get_params(lbm_nn n, float *params) {
+    if (!n || !params) return lbm::set_error(LBM_ERR_ARG, "null argument");
+    NN_TRY(hipSetDevice(n->device));
+    NN_TRY(hipStreamSynchronize(n->s));
+    NN_TRY(hipMemcpy(params, n->w, sizeof(float) * n->np, hipMemcpyDeviceToHost));
+    return LBM_OK;
+}
+
+int32_t lbm_nn_set_params(lbm_nn n, const float *params) {
+    if (!n || !params) return lbm::set_error(LBM_ERR_ARG, "null argument");
+    NN_TRY(hipSetDevice(n->device));
+    NN_TRY(hipStreamSynchronize(n->s));
+    NN_TRY(hipMemcpy(n->w, params, sizeof(float) * n->np, hipMemcpyHostToDevice));
+    NN_TRY(hipMemset(n->m, 0, sizeof(float) * n->np));
+    NN_TRY(hipMemset(n->v, 0, sizeof(float) * n->np));
+    n->t = 0;
+    return LBM_OK;
+}
+
+int32_t lbm_nn_forward(lbm_nn n, int32_t batch, const float *x, float *y) {
+    int32_t rc = check_net(n, batch);
+    if (rc) return rc;
+    if (!x || !y) return lbm::set_error(LBM_ERR_ARG, "null array");
+    if ((rc = forward(n, batch, x))) return rc;
+    const long P = (long)n->G * n->G;
+    hipLaunchKernelGGL(nn::k_to_smajor, dim3(nn::grid_for(3L * batch * P)), dim3(256), 0, n->s,
+                       (const float *)n->y, n->stage, batch, 3, P);
+    NN_TRY(hipMemcpyAsync(y, n->stage, sizeof(float) * 3 * batch * P, hipMemcpyDeviceToHost,
+                          n->s));
+    NN_TRY(hipStreamSynchronize(n->s));
+    return LBM_OK;
+}
+
+int32_t lbm_nn_loss_grad(lbm_nn n, int32_t batch, const float *x, const float *target,
+                         float *loss, float *grad) {
+    int32_t rc = check_net(n, batch);
+    if (rc) return rc;
+    if (!x || !target) return lbm::set_error(LBM_ERR_ARG, "null array");
+    if ((rc = forward(n, batch, x)) || (rc = backward(n, batch, target, loss))) return rc;
+    if (grad) NN_TRY(hipMemcpy(grad, n->g, sizeof(float) * n->np, hipMemcpyDeviceToHost));
+    return LBM_OK;
+}
+
+int32_t lbm_nn_train_step(lbm_nn n, int32_t batch, const float *x, const float *target,
+                          float lr, float *loss) {
+    int32_t rc = check_net(n, batch);
+    if (rc) return rc;
+    if (!x || !target) return lbm::set_error(LBM_ERR_ARG, "null array");
+    if ((rc = forward(n, batch, x)) || (rc = backward(n, batch, target, loss))) return rc;
+    n->t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - (float)std::pow(0.9, (double)n->t);
+    const float c2 = 1.0f - (float)std::pow(0.999, (double)n->t);
+    hipLaunchKernelGGL(nn::k_adam, dim3(nn::grid_for(n->np)), dim3(256), 0, n->s, n->w,
+                       (const float *)n->g, n->m, n->v, n->np, lr, b1, b2, 1e-8f, c1, c2);
+    NN_TRY(hipGetLastError());
+    return LBM_OK;
+}
+
+}  // extern "C"
