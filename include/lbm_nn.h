@@ -11,8 +11,10 @@
  *
  * Network: a convolutional encoder-decoder on a G x G node grid (G a
  * multiple of 8).  Input channels: log10(Ra), Pr, and the node coordinates
- * x/H and y/H (the stretched grid); output channels: u/U, v/U and T, with
- * U = sqrt(g beta H) the free-fall velocity of the cavity (LB units).
+ * x/H and y/H (the stretched grid), each mapped affinely onto [-1, 1] over
+ * its trained range by the caller; output channels: u/V, v/V and T, with
+ * V = sqrt(g beta H) / 4 a quarter of the cavity's free-fall velocity (LB
+ * units).
  *   encoder: 3x3 conv 4->16, 3x3 stride-2 convs 16->32->64->64 (ReLU)
  *   decoder: three stages of x2 nearest upsampling, concatenation with the
  *            encoder feature map of the same size, 3x3 conv (ReLU):

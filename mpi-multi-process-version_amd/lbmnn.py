@@ -3,9 +3,9 @@ liblbm_hip.so, and the helpers that turn cavity parameters into network
 inputs and network outputs into the fields lbm_init_fields() takes.
 
 Input channels: log10(Ra), Pr and the node coordinates x/H, y/H of the
-stretched grid; output channels: u/U, v/U and T with U = sqrt(g beta H), the
-cavity's free-fall velocity in LB units.  There is no fallback: a missing
-library or device raises.
+stretched grid; output channels: u/V, v/V and T with V = sqrt(g beta H) / 4,
+a quarter of the cavity's free-fall velocity in LB units.  There is no
+fallback: a missing library or device raises.
 """
 import ctypes
 
@@ -116,15 +116,32 @@ def free_fall_velocity(info):
     return float(np.sqrt(info["gbeta"] * info["H"]))
 
 
+def velocity_scale(info):
+    """V = U / 4, the velocity unit of the network's outputs: steady
+    velocities reach about U / 4, so u/V, v/V and T all span about [-1, 1]
+    and weigh alike in the squared error."""
+    return 0.25 * free_fall_velocity(info)
+
+
 def inputs(Ra, Pr, x, H):
-    """Network input [4][G][G] of one cavity: log10 Ra, Pr, x/H and y/H on
-    the node grid (x: node coordinates, H: cavity size)."""
+    """Network input [4][G][G] of one cavity, each channel mapped onto
+    [-1, 1] over the trained ranges: (log10 Ra - 4.5) / 1.5,
+    (Pr - 3.85) / 3.15, and 2 x/H - 1, 2 y/H - 1 on the node grid
+    (x: node coordinates, H: cavity size)."""
     G = len(x)
-    xi = np.asarray(x) / H
+    xi = 2.0 * np.asarray(x) / H - 1.0
     X, Y = np.meshgrid(xi, xi)   # [j][i]
-    return np.stack([np.full((G, G), np.log10(Ra)), np.full((G, G), Pr), X, Y]).astype(np.float32)
+    return np.stack([np.full((G, G), (np.log10(Ra) - 4.5) / 1.5),
+                     np.full((G, G), (Pr - 3.85) / 3.15), X, Y]).astype(np.float32)
 
 
-def targets(u, v, T, U):
-    """Network target [3][G][G] of one cavity's steady fields."""
-    return np.stack([u / U, v / U, T]).astype(np.float32)
+def targets(u, v, T, V):
+    """Network target [3][G][G] of one cavity's steady fields (V: the
+    velocity scale, velocity_scale())."""
+    return np.stack([u / V, v / V, T]).astype(np.float32)
+
+
+def fields(y, V):
+    """(u, v, T) in LB units from one network output y[3][G][G]."""
+    y = np.asarray(y, dtype=np.float64)
+    return y[0] * V, y[1] * V, y[2]

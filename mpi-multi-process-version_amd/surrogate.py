@@ -1,0 +1,314 @@
+"""Learned warm start of the steady 2D cavity: data from the batched solver,
+training of the HIP network (lbmnn.Net), and the warm-versus-cold
+comparison.
+
+The reference always starts from rest with the linear conduction profile
+(natural-convection_2dim_mono.f90:199-240) and marches to its stopping rule
+(errU, errT < eps on snapshots `check_every` steps apart,
+natural-convection_2dim_mono.f90:202-206,327).  Here the network predicts
+the steady (u, v, T) of a cavity from (Ra, Pr, grid) and lbm_init_fields()
+starts the solver from the equilibrium of that prediction; the solver then
+runs to the same stopping rule, so it remains the judge of the result and
+the cold start stays the fallback.
+
+Every step runs on the GPU: the batched solver generates the steady fields
+(one launch per time step for all cavities), the network trains with its
+own forward, backward and Adam kernels, and inference is one forward call.
+
+Command line (prints one JSON record per stage):
+    python surrogate.py train --out weights.npz [--count 2048 ...]
+    python surrogate.py compare --weights weights.npz [--count 100 ...]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+if _HERE not in sys.path:
+    sys.path.insert(0, _HERE)
+
+import lbmhip   # noqa: E402
+import lbmnn    # noqa: E402
+
+GRID = 64
+LOG_RA = (3.0, 6.0)
+PR = (0.7, 7.0)
+STRETCH = (1.0, 2.5)
+EPS = 1e-6
+CHECK_EVERY = 2000
+WEIGHTS = os.path.join(_HERE, "data", "warm64.npz")
+
+
+def sample_cavities(count, seed, n=GRID, log_ra=LOG_RA, pr=PR, stretch=STRETCH):
+    """Seeded cavity parameters: log10 Ra, Pr and the erf stretch factor
+    uniform in their ranges, all on an n x n grid."""
+    rng = np.random.default_rng(seed)
+    lr = rng.uniform(*log_ra, count)
+    p = rng.uniform(*pr, count)
+    a = rng.uniform(*stretch, count)
+    return [dict(n=n, Ra=float(10.0 ** lr[i]), Pr=float(p[i]), stretch=float(a[i]))
+            for i in range(count)]
+
+
+def _batch(cavities, eps, itc_max):
+    return lbmhip.Batch([dict(c, epsU=eps, epsT=eps, itc_max=itc_max) for c in cavities],
+                        check_every=CHECK_EVERY)
+
+
+def network_inputs(batch, cavities):
+    """x[b][4][G][G] of the cavities of a Batch (their own grids)."""
+    return np.stack([lbmnn.inputs(c["Ra"], c["Pr"], batch[i].grid(), batch[i].info()["H"])
+                     for i, c in enumerate(cavities)])
+
+
+def solve(cavities, eps=EPS, itc_max=600000, net=None, keep_fields=False):
+    """Runs every cavity to the stopping rule, from rest (net None) or from
+    the network's prediction.  Returns a dict of per-cavity arrays (itc,
+    status, errU, errT, Nu on the hot wall) and wall times in seconds:
+    `predict` (inputs, forward, lbm_init_fields), `run` (the time loop)."""
+    b = _batch(cavities, eps, itc_max)
+    t0 = time.perf_counter()
+    x = network_inputs(b, cavities)
+    if net is None:
+        b.init_rest()
+    else:
+        y = predict(net, x)
+        for i in range(len(cavities)):
+            u, v, T = lbmnn.fields(y[i], lbmnn.velocity_scale(b[i].info()))
+            b[i].init_fields(u, v, None, T)
+    b[0].synchronize()
+    t1 = time.perf_counter()
+    itc, eu, et, st = b.run()
+    t2 = time.perf_counter()
+    out = dict(itc=itc, status=st, errU=eu, errT=et, nu=np.array(b.nusselt()[0]),
+               predict_s=t1 - t0, run_s=t2 - t1, x=x)
+    if keep_fields:
+        t = np.zeros((len(cavities), 3, GRID, GRID), dtype=np.float32)
+        for i in range(len(cavities)):
+            _, u, v, _, T = b[i].fields()
+            t[i] = lbmnn.targets(u, v, T, lbmnn.velocity_scale(b[i].info()))
+        out["y"] = t
+    b.close()
+    return out
+
+
+def predict(net, x):
+    """Network output for any number of samples, in chunks of batch_max."""
+    return np.concatenate([net.forward(x[i:i + net.batch_max])
+                           for i in range(0, len(x), net.batch_max)])
+
+
+def generate(count, seed, eps=EPS, itc_max=600000, per_batch=256, keep_tol=1e-4, log=print):
+    """Steady fields of `count` seeded cavities.  A cavity is kept if it
+    reached the stopping rule, or ran to itc_max with errU and errT below
+    keep_tol (a slowly settling cavity, already steady to that tolerance);
+    diverged cavities are dropped.  Returns (x, y, record)."""
+    cavs = sample_cavities(count, seed)
+    xs, ys, keep, itcs = [], [], [], []
+    t0 = time.perf_counter()
+    for s in range(0, count, per_batch):
+        part = cavs[s:s + per_batch]
+        r = solve(part, eps, itc_max, keep_fields=True)
+        ok = (r["status"] == lbmhip.LBM_OK) | ((r["status"] == lbmhip.LBM_ERR_NOT_CONVERGED) &
+                                              (r["errU"] <= keep_tol) & (r["errT"] <= keep_tol))
+        xs.append(r["x"][ok])
+        ys.append(r["y"][ok])
+        keep += [s + i for i in np.nonzero(ok)[0]]
+        itcs.append(r["itc"])
+        log(json.dumps({"stage": "generate", "done": s + len(part), "of": count,
+                        "converged": int(ok.sum()), "max_itc": int(r["itc"].max()),
+                        "run_s": round(r["run_s"], 2)}))
+    itcs = np.concatenate(itcs)
+    rec = dict(count=count, seed=seed, kept=len(keep), eps=eps, itc_max=itc_max, keep_tol=keep_tol,
+               itc_median=float(np.median(itcs)), seconds=time.perf_counter() - t0)
+    return np.concatenate(xs), np.concatenate(ys), rec
+
+
+def train(net, x, y, epochs, batch=32, lr=2e-3, lr_min=2e-5, seed=0, val=None, log=print):
+    """Adam over shuffled mini-batches with a cosine learning-rate decay;
+    returns the per-epoch mean training loss (and validation loss)."""
+    rng = np.random.default_rng(seed)
+    hist = []
+    steps = epochs * ((len(x) + batch - 1) // batch)
+    k = 0
+    t0 = time.perf_counter()
+    for e in range(epochs):
+        order = rng.permutation(len(x))
+        tot = 0.0
+        for s in range(0, len(x), batch):
+            idx = np.sort(order[s:s + batch])
+            rate = lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * k / steps))
+            tot += net.train_step(x[idx], y[idx], rate) * len(idx)
+            k += 1
+        entry = {"epoch": e + 1, "loss": tot / len(x)}
+        if val is not None and ((e + 1) % 10 == 0 or e + 1 == epochs):
+            d = predict(net, val[0]) - val[1]
+            entry["val_loss"] = float(np.mean(d ** 2))
+            entry["val_rms"] = np.sqrt(np.mean(d ** 2, axis=(0, 2, 3))).round(5).tolist()
+        hist.append(entry)
+        if (e + 1) % 10 == 0 or e + 1 == epochs:
+            log(json.dumps(dict(stage="train", seconds=round(time.perf_counter() - t0, 1),
+                                **entry)))
+    return hist
+
+
+def save_weights(path, net, record):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=net.get_params(), grid=np.int32(net.grid),
+             record=np.array(json.dumps(record)))
+
+
+def load_net(path=WEIGHTS, batch_max=32, device=0):
+    """A network with the parameters saved by save_weights()."""
+    with np.load(path, allow_pickle=False) as z:
+        net = lbmnn.Net(grid=int(z["grid"]), batch_max=batch_max, device=device)
+        net.set_params(z["params"])
+    return net
+
+
+def traced_run(cavity, net=None, eps=EPS, itc_max=600000):
+    """One cavity to the stopping rule (the loop of lbm_run(): check every
+    CHECK_EVERY steps, stop when errU and errT are both <= eps), from rest
+    or from the network's prediction, recording after every check the step
+    count, the wall time since the start (inference and lbm_init_fields
+    included) and the hot-wall Nusselt number."""
+    c = dict(cavity)
+    s = lbmhip.Solver(2, c.pop("n"), c.pop("Ra"), Pr=c.pop("Pr"), stretch=c.pop("stretch"),
+                      epsU=eps, epsT=eps, check_every=CHECK_EVERY, itc_max=itc_max)
+    t0 = time.perf_counter()
+    if net is None:
+        s.init_rest()
+    else:
+        x = lbmnn.inputs(cavity["Ra"], cavity["Pr"], s.grid(), s.info()["H"])
+        u, v, T = lbmnn.fields(net.forward(x[None])[0], lbmnn.velocity_scale(s.info()))
+        s.init_fields(u, v, None, T)
+    status, trace = lbmhip.LBM_ERR_NOT_CONVERGED, []
+    while s.itc < itc_max:
+        s.step(min(CHECK_EVERY, itc_max - s.itc))
+        eu, et = s.check(allow_diverged=True)
+        if not (np.isfinite(eu) and np.isfinite(et)):
+            trace.append((s.itc, time.perf_counter() - t0, np.nan))
+            status = lbmhip.LBM_ERR_DIVERGED
+            break
+        nu = s.nusselt()[0]
+        trace.append((s.itc, time.perf_counter() - t0, nu))
+        if eu <= eps and et <= eps:
+            status = lbmhip.LBM_OK
+            break
+    s.close()
+    return dict(status=status, itc=trace[-1][0], wall=trace[-1][1], nu=trace[-1][2],
+                trace=np.array(trace))
+
+
+def band_entry(trace, nu_ref, tol=1e-3):
+    """(itc, wall) of the first check from which on every recorded Nusselt
+    number stays within tol (relative) of nu_ref; None if the last one is
+    outside."""
+    dev = np.abs(trace[:, 2] / nu_ref - 1.0)
+    if not np.isfinite(dev[-1]) or dev[-1] > tol:
+        return None
+    outside = np.nonzero(~(dev <= tol))[0]
+    k = 0 if outside.size == 0 else outside[-1] + 1
+    return trace[k, 0], trace[k, 1]
+
+
+def compare(net, cavities, eps=EPS, itc_max=600000, tol=1e-3, log=None):
+    """Cold and warm start of each cavity to the same stopping rule.
+
+    The cold run is the judge: only cavities whose cold run reaches the
+    stopping rule count.  A warm run that diverges or stops early falls
+    back to the cold start, and its time is charged in full (warm attempt
+    plus cold run).  Two times are compared: to the stopping rule, and to
+    the check from which on the hot-wall Nusselt number stays within tol
+    of the cold run's converged value."""
+    net.forward(np.zeros((1, 4, net.grid, net.grid), np.float32))   # load the kernels once
+    rows = []
+    for i, cav in enumerate(cavities):
+        cold = traced_run(cav, None, eps, itc_max)
+        warm = traced_run(cav, net, eps, itc_max)
+        row = dict(Ra=cav["Ra"], Pr=cav["Pr"], stretch=cav["stretch"],
+                   cold_status=cold["status"], warm_status=warm["status"],
+                   cold_itc=int(cold["itc"]), warm_itc=int(warm["itc"]),
+                   cold_s=cold["wall"], warm_s=warm["wall"],
+                   cold_nu=cold["nu"], warm_nu=warm["nu"])
+        if cold["status"] == lbmhip.LBM_OK:
+            cb = band_entry(cold["trace"], cold["nu"], tol)
+            wb = band_entry(warm["trace"], cold["nu"], tol)
+            fallback = warm["status"] != lbmhip.LBM_OK
+            if fallback:
+                row["warm_s"] = warm["wall"] + cold["wall"]
+                row["warm_itc"] = int(warm["itc"] + cold["itc"])
+                wb = (warm["itc"] + cb[0], warm["wall"] + cb[1])
+            row.update(fallback=fallback, nu_rel_dev=abs(warm["nu"] / cold["nu"] - 1.0)
+                       if not fallback else 0.0,
+                       cold_band_itc=int(cb[0]), cold_band_s=cb[1],
+                       warm_band_itc=int(wb[0]) if wb else None, warm_band_s=wb[1] if wb else None)
+        rows.append(row)
+        if log:
+            log(json.dumps(dict(stage="compare", cavity=i, **row)))
+    return rows, summarise(rows)
+
+
+def summarise(rows):
+    """Aggregate ratios over the cavities whose cold run converged."""
+    ok = [r for r in rows if r["cold_status"] == lbmhip.LBM_OK]
+    if not ok:
+        return dict(cavities=len(rows), judged=0)
+    banded = [r for r in ok if r["warm_band_s"] is not None]
+    tot = lambda k, rs: float(sum(r[k] for r in rs))
+    return dict(
+        cavities=len(rows), judged=len(ok),
+        fallbacks=sum(r["fallback"] for r in ok),
+        nu_rel_dev_max=max(r["nu_rel_dev"] for r in ok),
+        stop_wall_ratio=tot("cold_s", ok) / tot("warm_s", ok),
+        stop_itc_ratio=tot("cold_itc", ok) / tot("warm_itc", ok),
+        stop_wall_ratio_median=float(np.median([r["cold_s"] / r["warm_s"] for r in ok])),
+        band_cavities=len(banded),
+        band_wall_ratio=tot("cold_band_s", banded) / tot("warm_band_s", banded) if banded else None,
+        band_itc_ratio=tot("cold_band_itc", banded) / tot("warm_band_itc", banded) if banded else None,
+        band_wall_ratio_median=float(np.median([r["cold_band_s"] / r["warm_band_s"]
+                                                for r in banded])) if banded else None,
+        cold_s=tot("cold_s", ok), warm_s=tot("warm_s", ok))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", default=WEIGHTS)
+    t.add_argument("--count", type=int, default=2048)
+    t.add_argument("--seed", type=int, default=20261203)
+    t.add_argument("--epochs", type=int, default=200)
+    t.add_argument("--batch", type=int, default=32)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--itc-max", type=int, default=600000)
+    t.add_argument("--per-batch", type=int, default=256)
+    c = sub.add_parser("compare")
+    c.add_argument("--weights", default=WEIGHTS)
+    c.add_argument("--count", type=int, default=100)
+    c.add_argument("--seed", type=int, default=7)
+    c.add_argument("--itc-max", type=int, default=600000)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    if a.cmd == "train":
+        x, y, rec = generate(a.count, a.seed, itc_max=a.itc_max, per_batch=a.per_batch, log=log)
+        nval = max(1, len(x) // 16)
+        net = lbmnn.Net(grid=GRID, batch_max=a.batch, seed=a.seed)
+        hist = train(net, x[nval:], y[nval:], a.epochs, a.batch, a.lr,
+                     val=(x[:nval], y[:nval]), log=log)
+        rec.update(epochs=a.epochs, batch=a.batch, lr=a.lr, train_samples=len(x) - nval,
+                   val_samples=nval, final=hist[-1])
+        save_weights(a.out, net, rec)
+        log(json.dumps(dict(stage="saved", path=a.out, **rec)))
+    else:
+        net = load_net(a.weights)
+        _, summary = compare(net, sample_cavities(a.count, a.seed), itc_max=a.itc_max, log=log)
+        log(json.dumps(dict(stage="summary", seed=a.seed, **summary)))
+
+
+if __name__ == "__main__":
+    main()
