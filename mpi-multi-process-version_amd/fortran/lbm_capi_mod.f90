@@ -210,6 +210,58 @@ module lbm_capi
             integer(c_int32_t), value :: steps_per_trial
             real(c_double), intent(out) :: best_ms
         end function
+        ! the learned warm start (include/lbm_nn.h): arrays are FP32,
+        ! x(G, G, 4, batch) and y(G, G, 3, batch) in Fortran order
+        integer(c_int32_t) function lbm_nn_create(grid, batch_max, seed, device, net) &
+                bind(C, name='lbm_nn_create')
+            import :: c_int32_t, c_int64_t, c_ptr
+            integer(c_int32_t), value :: grid, batch_max, device
+            integer(c_int64_t), value :: seed
+            type(c_ptr), intent(out) :: net
+        end function
+        integer(c_int32_t) function lbm_nn_destroy(net) bind(C, name='lbm_nn_destroy')
+            import :: c_int32_t, c_ptr
+            type(c_ptr), value :: net
+        end function
+        integer(c_int32_t) function lbm_nn_param_count(net, count) bind(C, name='lbm_nn_param_count')
+            import :: c_int32_t, c_int64_t, c_ptr
+            type(c_ptr), value :: net
+            integer(c_int64_t), intent(out) :: count
+        end function
+        integer(c_int32_t) function lbm_nn_get_params(net, params) bind(C, name='lbm_nn_get_params')
+            import :: c_int32_t, c_ptr, c_float
+            type(c_ptr), value :: net
+            real(c_float), intent(out) :: params(*)
+        end function
+        integer(c_int32_t) function lbm_nn_set_params(net, params) bind(C, name='lbm_nn_set_params')
+            import :: c_int32_t, c_ptr, c_float
+            type(c_ptr), value :: net
+            real(c_float), intent(in) :: params(*)
+        end function
+        integer(c_int32_t) function lbm_nn_forward(net, batch, x, y) bind(C, name='lbm_nn_forward')
+            import :: c_int32_t, c_ptr, c_float
+            type(c_ptr), value :: net
+            integer(c_int32_t), value :: batch
+            real(c_float), intent(in) :: x(*)
+            real(c_float), intent(out) :: y(*)
+        end function
+        integer(c_int32_t) function lbm_nn_loss_grad(net, batch, x, target, loss, grad) &
+                bind(C, name='lbm_nn_loss_grad')
+            import :: c_int32_t, c_ptr, c_float
+            type(c_ptr), value :: net
+            integer(c_int32_t), value :: batch
+            real(c_float), intent(in) :: x(*), target(*)
+            real(c_float), intent(out) :: loss, grad(*)
+        end function
+        integer(c_int32_t) function lbm_nn_train_step(net, batch, x, target, lr, loss) &
+                bind(C, name='lbm_nn_train_step')
+            import :: c_int32_t, c_ptr, c_float
+            type(c_ptr), value :: net
+            integer(c_int32_t), value :: batch
+            real(c_float), intent(in) :: x(*), target(*)
+            real(c_float), value :: lr
+            real(c_float), intent(out) :: loss
+        end function
         type(c_ptr) function lbm_status_string(status) bind(C, name='lbm_status_string')
             import :: c_ptr, c_int32_t
             integer(c_int32_t), value :: status

@@ -124,7 +124,7 @@ def generate(count, seed, eps=EPS, itc_max=600000, per_batch=256, keep_tol=1e-4,
                         "run_s": round(r["run_s"], 2)}))
     itcs = np.concatenate(itcs)
     rec = dict(count=count, seed=seed, kept=len(keep), eps=eps, itc_max=itc_max, keep_tol=keep_tol,
-               itc_median=float(np.median(itcs)), seconds=time.perf_counter() - t0)
+               itc_median=float(np.median(itcs)), generate_seconds=time.perf_counter() - t0)
     return np.concatenate(xs), np.concatenate(ys), rec
 
 
@@ -272,7 +272,8 @@ def summarise(rows):
         band_itc_ratio=tot("cold_band_itc", banded) / tot("warm_band_itc", banded) if banded else None,
         band_wall_ratio_median=float(np.median([r["cold_band_s"] / r["warm_band_s"]
                                                 for r in banded])) if banded else None,
-        cold_s=tot("cold_s", ok), warm_s=tot("warm_s", ok))
+        cold_s=tot("cold_s", ok), warm_s=tot("warm_s", ok),
+        cold_band_s=tot("cold_band_s", banded), warm_band_s=tot("warm_band_s", banded))
 
 
 def main(argv=None):
@@ -287,6 +288,7 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--itc-max", type=int, default=600000)
     t.add_argument("--per-batch", type=int, default=256)
+    t.add_argument("--keep-tol", type=float, default=1e-4)
     c = sub.add_parser("compare")
     c.add_argument("--weights", default=WEIGHTS)
     c.add_argument("--count", type=int, default=100)
@@ -295,13 +297,15 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     if a.cmd == "train":
-        x, y, rec = generate(a.count, a.seed, itc_max=a.itc_max, per_batch=a.per_batch, log=log)
+        x, y, rec = generate(a.count, a.seed, itc_max=a.itc_max, per_batch=a.per_batch,
+                             keep_tol=a.keep_tol, log=log)
         nval = max(1, len(x) // 16)
         net = lbmnn.Net(grid=GRID, batch_max=a.batch, seed=a.seed)
+        t0 = time.perf_counter()
         hist = train(net, x[nval:], y[nval:], a.epochs, a.batch, a.lr,
                      val=(x[:nval], y[:nval]), log=log)
         rec.update(epochs=a.epochs, batch=a.batch, lr=a.lr, train_samples=len(x) - nval,
-                   val_samples=nval, final=hist[-1])
+                   val_samples=nval, final=hist[-1], train_seconds=time.perf_counter() - t0)
         save_weights(a.out, net, rec)
         log(json.dumps(dict(stage="saved", path=a.out, **rec)))
     else:

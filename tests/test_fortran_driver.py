@@ -43,6 +43,21 @@ def _run(tmp_path, timeout=120, **kw):
 
 # --------------------------------------------------------------- CPU tests
 
+def test_fortran_module_binds_every_c_entry_point():
+    """fortran/lbm_capi_mod.f90 has a bind(C) interface for every function
+    declared in include/*.h, under the same name."""
+    decl = re.compile(r"^\s*(?:int32_t|lbm_handle|const char \*)\s*\*?\s*(lbm_\w+)\s*\(", re.M)
+    names = set()
+    for h in ("lbm_capi.h", "lbm_nn.h"):
+        names |= set(decl.findall(open(os.path.join(ROOT, "include", h)).read()))
+    mod = open(os.path.join(ROOT, "mpi-multi-process-version_amd", "fortran",
+                            "lbm_capi_mod.f90")).read()
+    bound = set(re.findall(r"bind\(C,\s*name='(lbm_\w+)'\)", mod))
+    assert len(names) > 40
+    assert names == bound, (sorted(names - bound), sorted(bound - names))
+
+
+
 def test_rejects_bad_grid_before_touching_the_gpu(tmp_path):
     r = _run(tmp_path, dim=2, nx=3, Ra=1e4)
     assert r.returncode == 1
