@@ -114,9 +114,14 @@ int32_t batch_steps(lbm_batch b, int64_t nsteps) {
         int32_t rc = rebuild(b);
         if (rc) return rc;
     }
+    double nodes = 0.0;   // active nodes: 14 FP32 populations read and written per node
+    for (size_t c = 0; c < b->cav.size(); c++)
+        if (b->active[c]) nodes += (double)b->cav[c]->nn;
+    const bool nt = lbm::stream_stores(nodes * 14 * 4.0 * 2.0);
     for (int64_t s = 0; s < nsteps; s++) {
         hipError_t e = lbm::launch_step_batch_f32(s == nsteps - 1, b->d_args, b->d_tiles,
-                                                  b->ntiles, b->parity, b->bx, b->by, b->stream);
+                                                  b->ntiles, b->parity, b->bx, b->by, nt,
+                                                  b->stream);
         if (e != hipSuccess) return hip_err(e, "batched step launch");
         b->parity ^= 1;
     }

@@ -61,6 +61,13 @@ struct Ctx {
 // current buffers are set by the caller).
 void fill_args(Ctx &c);
 
+// Capacity of the memory-side last-level cache (MALL, Infinity Cache) of
+// MI355X.  A step whose population traffic (read + write) exceeds it runs
+// with streaming stores; below it, ordinary stores keep the written planes
+// in the MALL for the next step.
+constexpr double kMallBytes = 256.0 * 1024 * 1024;
+inline bool stream_stores(double bytes_per_step) { return bytes_per_step > kMallBytes; }
+
 // Launchers (lbm_step_f32.hip / lbm_step_f64.hip).
 hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a,
                            const LaunchShape &ls, hipStream_t s);
@@ -81,7 +88,7 @@ hipError_t launch_nusselt(const Ctx &c);    // results: d_red[4..5]
 // the cavity at parity 0); workgroup b uses args[tiles[b].w ^ parity].  2D,
 // FP32 shifted storage.
 hipError_t launch_step_batch_f32(bool macro, const StepArgs<float> *args, const int4 *tiles,
-                                 int ntiles, int parity, int bx, int by, hipStream_t s);
+                                 int ntiles, int parity, int bx, int by, bool nt, hipStream_t s);
 
 constexpr int kRedThreads = 256;
 constexpr int kRedMaxBlocks = 1024;

@@ -431,9 +431,14 @@ __device__ __forceinline__ R ldp(const R *__restrict__ base, uint32_t idx) {
     return *reinterpret_cast<const R *>(reinterpret_cast<const char *>(base) +
                                         idx * (uint32_t)sizeof(R));
 }
-template <typename R>
+// NT: a streaming (non-temporal) store, for states larger than the MALL:
+// the written planes are not read again within the step, so they should not
+// displace the rows that neighbouring workgroups are about to re-read.
+template <bool NT = false, typename R>
 __device__ __forceinline__ void stp(R *__restrict__ base, uint32_t idx, R v) {
-    *reinterpret_cast<R *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(R)) = v;
+    R *p = reinterpret_cast<R *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(R));
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
 // One population: a 1D interpolation when one velocity component is
@@ -524,7 +529,7 @@ __device__ __forceinline__ HeatShift<R> load_heat_shift(const StepArgs<R, S> &a,
     return h;
 }
 
-template <int DIM, typename S, typename R, bool SHIFT, bool MACRO>
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, bool NT = false>
 __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node,
                                           const int (&idx)[3], const R *pcf, const R *pcg,
                                           R *f, R *g, R sp_f, R sp_g, const HeatShift<R> &hs) {
@@ -659,17 +664,17 @@ __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node
 
     // 4. collision with the force of this step
     const R Fb = a.gbeta * T * rho;
-    if constexpr (EXACT) stp(a.force, node, Fb);
+    if constexpr (EXACT) stp<NT>(a.force, node, Fb);
     R fp[QF], gp[QG];
     collide_flow<DIM, R, EXACT>(f, SHIFT ? srho : rho, rho, ux, uy, uz, Fb, a, fp);
     collide_heat<DIM, R, EXACT>(g, SHIFT ? sT : T, T, ux, uy, uz, a, gp);
     static_for<0, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        stp(a.pout + al * a.plane, node, S(fp[al]));
+        stp<NT>(a.pout + al * a.plane, node, S(fp[al]));
     });
     static_for<0, QG>([&](auto A) {
         constexpr int al = decltype(A)::value;
-        stp(a.pout + (QF + al) * a.plane, node, S(gp[al]));
+        stp<NT>(a.pout + (QF + al) * a.plane, node, S(gp[al]));
     });
 }
 
@@ -677,7 +682,7 @@ __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node
 // then caps the registers per lane), trading load hoisting for latency
 // hiding; the autotuner picks.
 // One node (i, j, k) of the fused step (the node must be inside the grid).
-template <int DIM, typename S, typename R, bool SHIFT, bool MACRO>
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, bool NT = false>
 __device__ __forceinline__ void step_node(const StepArgs<R, S> &a, int i, int j, int k) {
     constexpr bool EXACT = std::is_same<S, double>::value;
     using FL = Flow<DIM>;
@@ -717,11 +722,11 @@ __device__ __forceinline__ void step_node(const StepArgs<R, S> &a, int i, int j,
         g[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2], EXACT>(
             a.pin + (QF + al) * a.plane, node, pcg[al], st, stride);
     });
-    node_tail<DIM, S, R, SHIFT, MACRO>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
-                                       load_heat_shift<DIM, SHIFT>(a, idx));
+    node_tail<DIM, S, R, SHIFT, MACRO, NT>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
+                                           load_heat_shift<DIM, SHIFT>(a, idx));
 }
 
-template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, int WPE = 0>
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, int WPE = 0, bool NT = false>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_step(StepArgs<R, S> a) {
     int tx, ty, tz;
@@ -730,7 +735,7 @@ void k_step(StepArgs<R, S> a) {
     const int j = ty * blockDim.y + threadIdx.y;
     const int k = DIM == 3 ? tz : 0;
     if (i >= a.n || j >= a.n) return;
-    step_node<DIM, S, R, SHIFT, MACRO>(a, i, j, k);
+    step_node<DIM, S, R, SHIFT, MACRO, NT>(a, i, j, k);
 }
 
 // Initial collision (iteration 1's collision_U/collision_T): equilibrium

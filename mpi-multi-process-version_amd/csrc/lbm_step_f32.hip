@@ -7,13 +7,27 @@
 
 namespace lbm {
 
-template <int DIM, int WPE>
+template <int DIM, int WPE, bool NT>
 static void launch_k(bool macro, dim3 grid, dim3 block, const StepArgs<float> &a,
                      hipStream_t s) {
     if (macro)
-        hipLaunchKernelGGL((k_step<DIM, float, float, true, true, WPE>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((k_step<DIM, float, float, true, true, WPE, NT>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL((k_step<DIM, float, float, true, false, WPE>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((k_step<DIM, float, float, true, false, WPE, NT>), grid, block, 0, s,
+                           a);
+}
+
+template <int DIM, int WPE>
+static void launch_k(bool macro, bool nt, dim3 grid, dim3 block, const StepArgs<float> &a,
+                     hipStream_t s) {
+    if (nt) launch_k<DIM, WPE, true>(macro, grid, block, a, s);
+    else launch_k<DIM, WPE, false>(macro, grid, block, a, s);
+}
+
+// population bytes read and written by one FP32 step
+static double step_bytes_f32(int dim, int n) {
+    const double nodes = dim == 2 ? (double)n * n : (double)n * n * n;
+    return nodes * (dim == 2 ? 14 : 26) * 4.0 * 2.0;
 }
 
 hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
@@ -32,10 +46,11 @@ hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a0, const
     a.ntiles = a.gx * a.gy * (dim == 3 ? a.n : 1);
     a.remap = (ls.remap == 2 && a.gy % 8) ? 0 : ls.remap;
     dim3 grid(a.ntiles);
-    if (dim == 2) launch_k<2, 0>(macro, grid, block, a, s);
-    else if (ls.occ == 5) launch_k<3, 5>(macro, grid, block, a, s);
-    else if (ls.occ == 6) launch_k<3, 6>(macro, grid, block, a, s);
-    else launch_k<3, 0>(macro, grid, block, a, s);
+    const bool nt = stream_stores(step_bytes_f32(dim, a.n));
+    if (dim == 2) launch_k<2, 0>(macro, nt, grid, block, a, s);
+    else if (ls.occ == 5) launch_k<3, 5>(macro, nt, grid, block, a, s);
+    else if (ls.occ == 6) launch_k<3, 6>(macro, nt, grid, block, a, s);
+    else launch_k<3, 0>(macro, nt, grid, block, a, s);
     return hipGetLastError();
 }
 
@@ -49,6 +64,15 @@ int zm_planes_per_group(int n, int by) {
     return (int)((n + nzc - 1) / nzc);
 }
 
+template <int BY>
+static void launch_zm(bool macro, bool nt, dim3 grid, dim3 block, const StepArgs<float> &a, int kc,
+                      hipStream_t s) {
+    if (macro && nt) hipLaunchKernelGGL((k_step_zm<BY, true, true>), grid, block, 0, s, a, kc);
+    else if (macro) hipLaunchKernelGGL((k_step_zm<BY, true, false>), grid, block, 0, s, a, kc);
+    else if (nt) hipLaunchKernelGGL((k_step_zm<BY, false, true>), grid, block, 0, s, a, kc);
+    else hipLaunchKernelGGL((k_step_zm<BY, false, false>), grid, block, 0, s, a, kc);
+}
+
 hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
                           hipStream_t s) {
     if (ls.bx != 64 || (ls.by != 2 && ls.by != 4 && ls.by != 8)) return hipErrorInvalidValue;
@@ -59,20 +83,14 @@ hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchSha
     a.ntiles = a.gx * a.gy * ((a.n + kc - 1) / kc);
     a.remap = (ls.remap == 2 && a.gy % 8) ? 0 : ls.remap;
     dim3 block(64, ls.by, 1), grid(a.ntiles);
-    if (ls.by == 2) {
-        if (macro) hipLaunchKernelGGL((k_step_zm<2, true>), grid, block, 0, s, a, kc);
-        else hipLaunchKernelGGL((k_step_zm<2, false>), grid, block, 0, s, a, kc);
-    } else if (ls.by == 8) {
-        if (macro) hipLaunchKernelGGL((k_step_zm<8, true>), grid, block, 0, s, a, kc);
-        else hipLaunchKernelGGL((k_step_zm<8, false>), grid, block, 0, s, a, kc);
-    } else {
-        if (macro) hipLaunchKernelGGL((k_step_zm<4, true>), grid, block, 0, s, a, kc);
-        else hipLaunchKernelGGL((k_step_zm<4, false>), grid, block, 0, s, a, kc);
-    }
+    const bool nt = stream_stores(step_bytes_f32(3, a.n));
+    if (ls.by == 2) launch_zm<2>(macro, nt, grid, block, a, kc, s);
+    else if (ls.by == 8) launch_zm<8>(macro, nt, grid, block, a, kc, s);
+    else launch_zm<4>(macro, nt, grid, block, a, kc, s);
     return hipGetLastError();
 }
 
-template <bool MACRO>
+template <bool MACRO, bool NT>
 __global__ __launch_bounds__(1024) void k_step_batch(const StepArgs<float> *__restrict__ args,
                                                      const int4 *__restrict__ tiles, int parity) {
     const int4 t = tiles[blockIdx.x];
@@ -80,15 +98,21 @@ __global__ __launch_bounds__(1024) void k_step_batch(const StepArgs<float> *__re
     const int i = t.y * (int)blockDim.x + (int)threadIdx.x;
     const int j = t.z * (int)blockDim.y + (int)threadIdx.y;
     if (i >= a.n || j >= a.n) return;
-    step_node<2, float, float, true, MACRO>(a, i, j, 0);
+    step_node<2, float, float, true, MACRO, NT>(a, i, j, 0);
 }
 
 hipError_t launch_step_batch_f32(bool macro, const StepArgs<float> *args, const int4 *tiles,
-                                 int ntiles, int parity, int bx, int by, hipStream_t s) {
+                                 int ntiles, int parity, int bx, int by, bool nt, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
     dim3 block(bx, by, 1), grid(ntiles);
-    if (macro) hipLaunchKernelGGL(k_step_batch<true>, grid, block, 0, s, args, tiles, parity);
-    else hipLaunchKernelGGL(k_step_batch<false>, grid, block, 0, s, args, tiles, parity);
+    if (macro && nt)
+        hipLaunchKernelGGL((k_step_batch<true, true>), grid, block, 0, s, args, tiles, parity);
+    else if (macro)
+        hipLaunchKernelGGL((k_step_batch<true, false>), grid, block, 0, s, args, tiles, parity);
+    else if (nt)
+        hipLaunchKernelGGL((k_step_batch<false, true>), grid, block, 0, s, args, tiles, parity);
+    else
+        hipLaunchKernelGGL((k_step_batch<false, false>), grid, block, 0, s, args, tiles, parity);
     return hipGetLastError();
 }
 

@@ -6,6 +6,22 @@
 
 namespace lbm {
 
+template <int DIM, bool NT>
+static void launch_k(bool macro, dim3 grid, dim3 block, const StepArgs<double> &a, hipStream_t s) {
+    if (macro)
+        hipLaunchKernelGGL((k_step<DIM, double, double, false, true, 0, NT>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_step<DIM, double, double, false, false, 0, NT>), grid, block, 0, s,
+                           a);
+}
+
+template <int DIM>
+static void launch_k(bool macro, bool nt, dim3 grid, dim3 block, const StepArgs<double> &a,
+                     hipStream_t s) {
+    if (nt) launch_k<DIM, true>(macro, grid, block, a, s);
+    else launch_k<DIM, false>(macro, grid, block, a, s);
+}
+
 hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a0, const LaunchShape &ls,
                            hipStream_t s) {
     StepArgs<double> a = a0;
@@ -15,13 +31,11 @@ hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a0, cons
     a.ntiles = a.gx * a.gy * (dim == 3 ? a.n : 1);
     a.remap = (ls.remap == 2 && a.gy % 8) ? 0 : ls.remap;
     dim3 grid(a.ntiles);
-    if (dim == 2) {
-        if (macro) hipLaunchKernelGGL((k_step<2, double, double, false, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step<2, double, double, false, false>), grid, block, 0, s, a);
-    } else {
-        if (macro) hipLaunchKernelGGL((k_step<3, double, double, false, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_step<3, double, double, false, false>), grid, block, 0, s, a);
-    }
+    // populations and the per-node force, read and written
+    const double nodes = dim == 2 ? (double)a.n * a.n : (double)a.n * a.n * a.n;
+    const bool nt = stream_stores(nodes * ((dim == 2 ? 14 : 26) + 1) * 8.0 * 2.0);
+    if (dim == 2) launch_k<2>(macro, nt, grid, block, a, s);
+    else launch_k<3>(macro, nt, grid, block, a, s);
     return hipGetLastError();
 }
 

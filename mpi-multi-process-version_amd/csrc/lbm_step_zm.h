@@ -246,7 +246,7 @@ __device__ __forceinline__ float zm_interp(const float *lds, int l, int r, float
 
 __device__ __forceinline__ int zm_window(int k, int n) { return min(max(k - 1, 0), n - 3); }
 
-template <int BY, bool MACRO>
+template <int BY, bool MACRO, bool NT>
 __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) {
     using L = ZmLayout<BY>;
     using FL = Flow<3>;
@@ -332,7 +332,8 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) 
             g[al] = zm_interp<BY, QF + al>(lds, l, r, pcg[al], st, zsl);
         });
         if (live)
-            node_tail<3, float, float, true, MACRO>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g, hs);
+            node_tail<3, float, float, true, MACRO, NT>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
+                                                        hs);
 
         __syncthreads();   // every read of plane k's staging is done
         if (more) zm_commit<BY, false>(lds, pin_, np, 0);
