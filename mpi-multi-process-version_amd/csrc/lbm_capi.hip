@@ -638,8 +638,8 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes) {
 // inside the staged halo.  Its staging offsets are 32-bit element offsets
 // over the 26 planes (26 n^3 < 2^32: n <= 544).
 static bool zm_launch_ok(const lbm_ctx &c, int bx, int by, int occ) {
-    return c.dim == 3 && !c.fp64 && bx == 64 && (by == 2 || by == 4 || by == 8) && occ == 0 &&
-           c.n % 4 == 0 && c.n <= 544;
+    return c.dim == 3 && !c.fp64 && bx == 64 && (by == 2 || by == 4 || by == 6 || by == 8) &&
+           occ == 0 && c.n % 4 == 0 && c.n <= 544;
 }
 
 int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
@@ -651,7 +651,7 @@ int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
         return fail(LBM_ERR_ARG, "invalid launch configuration");
     if (l->variant == 1 && !zm_launch_ok(*h, l->bx, l->by, l->occupancy))
         return fail(LBM_ERR_ARG,
-                    "variant 1 needs 3D FP32 storage, bx = 64, by = 2, 4 or 8, occupancy 0, "
+                    "variant 1 needs 3D FP32 storage, bx = 64, by = 2, 4, 6 or 8, occupancy 0, "
                     "n % 4 == 0 and n <= 544");
     h->shape.bx = l->bx;
     h->shape.by = l->by;
@@ -698,7 +698,7 @@ int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms) {
     const lbm::LaunchShape start = h->shape;
     // candidates (bx, by), each with every tile order and, for the 3D FP32
     // kernel, each occupancy target
-    static const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {64, 8}, {128, 1},
+    static const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {64, 6}, {64, 8}, {128, 1},
                                   {128, 2}, {128, 4}, {256, 1}, {256, 2}, {32, 8}};
     lbm::LaunchShape best = start;
     double best_t = 1e300;

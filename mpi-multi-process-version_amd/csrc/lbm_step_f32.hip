@@ -75,16 +75,21 @@ static void launch_zm(bool macro, bool nt, dim3 grid, dim3 block, const StepArgs
 
 hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
                           hipStream_t s) {
-    if (ls.bx != 64 || (ls.by != 2 && ls.by != 4 && ls.by != 8)) return hipErrorInvalidValue;
+    if (ls.bx != 64 || (ls.by != 2 && ls.by != 4 && ls.by != 6 && ls.by != 8))
+        return hipErrorInvalidValue;
     StepArgs<float> a = a0;
     const int kc = zm_planes_per_group(a.n, ls.by);
+    const int nzc = (a.n + kc - 1) / kc;
     a.gx = (a.n + 63) / 64;
     a.gy = (a.n + ls.by - 1) / ls.by;
-    a.ntiles = a.gx * a.gy * ((a.n + kc - 1) / kc);
-    a.remap = (ls.remap == 2 && a.gy % 8) ? 0 : ls.remap;
+    // remap 2 splits the tile rows into 8 XCD bands (k_step_zm's zm_tile);
+    // every XCD gets the largest band's share of workgroups
+    a.remap = (ls.remap == 2 && a.gy < 8) ? 0 : ls.remap;
+    a.ntiles = a.remap == 2 ? 8 * a.gx * ((a.gy + 7) / 8) * nzc : a.gx * a.gy * nzc;
     dim3 block(64, ls.by, 1), grid(a.ntiles);
     const bool nt = stream_stores(step_bytes_f32(3, a.n));
     if (ls.by == 2) launch_zm<2>(macro, nt, grid, block, a, kc, s);
+    else if (ls.by == 6) launch_zm<6>(macro, nt, grid, block, a, kc, s);
     else if (ls.by == 8) launch_zm<8>(macro, nt, grid, block, a, kc, s);
     else launch_zm<4>(macro, nt, grid, block, a, kc, s);
     return hipGetLastError();

@@ -5,21 +5,31 @@
 // stencil rows it shares with other workgroups (the y halo and the planes
 // k +- 1) are L1 misses, and the L1's miss handling, not HBM, bounds it.
 // Here a workgroup owns a 64 x BY column of nodes over a range of planes and
-// marches it in +z.  The post-collision populations it needs are staged in
-// LDS once per plane:
+// marches it in +z.  Each plane's post-collision populations are staged in
+// LDS once, one slot per population:
 //   * populations with c_z = 0 (except the rest ones): plane k, with a one-
-//     node halo in x if c_x != 0 and in y if c_y != 0;
-//   * populations with c_z != 0: a ring of three planes, the window
-//     [ws, ws + 2] with ws = clamp(k - 1, 0, n - 3), which holds every z
-//     stencil point of plane k, one-sided wall stencils included;
+//     node halo in x if c_x != 0 and in y if c_y != 0, interpolated from
+//     LDS when node k is computed;
+//   * populations with c_z != 0 and an in-plane component: the newest plane
+//     of the z window, with their x or y halo.  As a plane arrives, each
+//     lane interpolates it in-plane at its own (i, j) and keeps the result
+//     (and its own value) in a three-entry register ring; node k then
+//     interpolates along z from the ring.  The tensor-product weights are
+//     those of k_step, applied in-plane first;
+//   * populations with c = (0, 0, +-1) have no in-plane neighbours: each
+//     lane loads its own value of the arriving plane into the ring;
 //   * the two rest populations are read by each lane directly.
+// The ring holds the planes of the window [ws, ws + 2], ws = clamp(k - 1,
+// 0, n - 3), in order: a step to the next plane shifts it and interpolates
+// the arriving plane, except at the walls, where the window does not move.
+// The wall planes k = 0 and k = n-1 (one-sided z stencils) run their own
+// copies of the plane body, which take the stencil points from the ring in
+// their order; the hot copy takes them in window order and always shifts.
 // Rows are staged four at a time: a lane loads four consecutive x values
 // (one 16-byte load), so a wave moves four 64-wide rows per instruction, and
-// the rows of the next plane are fetched into registers while the current
-// plane is computed.  Every stencil read is an LDS read.  The interpolation
-// expressions and their operands are those of k_step; the results agree
-// with it to FP32 rounding (the compiler fuses a different handful of
-// multiply-adds in the two kernels).
+// the rows of the next planes are fetched into registers while the current
+// plane is computed.  The results agree with k_step to FP32 rounding (the
+// order of the tensor-product stages and the fused multiply-adds differ).
 //
 // All waves run one copy of the loop: each thread's staging offsets are
 // computed once (a plan) and the loads and LDS writes of a plane are
@@ -40,11 +50,11 @@ namespace lbm {
 typedef float zm_f4 __attribute__((ext_vector_type(4)));
 typedef float zm_f2 __attribute__((ext_vector_type(2)));
 
-// Staging slots p = 0..25: flow populations 0..18, then thermal 0..6
-// (whose velocities are the first seven flow velocities).  The population
-// plane of slot p is plane p of the buffer.  A staged region is H rows of
-// pitch P: x columns i0 .. i0+63 at C0 .. C0+63, and for populations with
-// an x halo, column i0-1 at 3 and column i0+64 at 68 (so rows stay 16-byte
+// Slots p = 0..25: flow populations 0..18, then thermal 0..6 (whose
+// velocities are the first seven flow velocities).  The population plane
+// of slot p is plane p of the buffer.  A staged region is H rows of pitch
+// P: x columns i0 .. i0+63 at C0 .. C0+63, and for populations with an x
+// halo, column i0-1 at 3 and column i0+64 at 68 (so rows stay 16-byte
 // aligned).
 template <int BY> struct ZmLayout {
     static constexpr int NP = 26;
@@ -52,26 +62,46 @@ template <int BY> struct ZmLayout {
     static constexpr int cx(int p) { return Flow<3>::c[vel(p)][0]; }
     static constexpr int cy(int p) { return Flow<3>::c[vel(p)][1]; }
     static constexpr int cz(int p) { return Flow<3>::c[vel(p)][2]; }
-    static constexpr bool staged(int p) { return p != 0 && p != 19; }
     static constexpr bool zpop(int p) { return cz(p) != 0; }
+    static constexpr bool purez(int p) { return zpop(p) && cx(p) == 0 && cy(p) == 0; }
+    static constexpr bool staged(int p) { return p != 0 && p != 19 && !purez(p); }
     static constexpr int hx(int p) { return cx(p) != 0 ? 1 : 0; }
     static constexpr int hy(int p) { return cy(p) != 0 ? 1 : 0; }
     static constexpr int P(int p) { return hx(p) ? 72 : 64; }
     static constexpr int C0(int p) { return hx(p) ? 4 : 0; }
     static constexpr int H(int p) { return BY + 2 * hy(p); }
-    static constexpr int slots(int p) { return zpop(p) ? 3 : 1; }
     static constexpr int psize(int p) { return staged(p) ? P(p) * H(p) : 0; }
     static constexpr int off(int p) {
         int o = 0;
-        for (int q = 0; q < p; q++) o += psize(q) * slots(q);
+        for (int q = 0; q < p; q++) o += psize(q);
         return o;
     }
     static constexpr int total() { return off(NP); }
-    // Rows of a phase (Z: the c_z != 0 populations at a new plane; !Z: the
-    // in-plane populations at plane k), in slot order.  Staging instruction
-    // q of wave w gives lane group g (16 lanes of 4 x values each) row
-    // R = 4 (q BY + w) + g, clamped to the last row (a repeated row writes
-    // the same values).
+    // register ring indices: over the c_z != 0 populations, over the staged
+    // ones among them, and over the (0, 0, +-1) ones
+    static constexpr int zi(int p) {
+        int c = 0;
+        for (int q = 0; q < p; q++) c += zpop(q) ? 1 : 0;
+        return c;
+    }
+    static constexpr int oi(int p) {
+        int c = 0;
+        for (int q = 0; q < p; q++) c += (zpop(q) && staged(q)) ? 1 : 0;
+        return c;
+    }
+    static constexpr int pzi(int p) {
+        int c = 0;
+        for (int q = 0; q < p; q++) c += purez(q) ? 1 : 0;
+        return c;
+    }
+    static constexpr int nz() { return zi(NP); }
+    static constexpr int nzo() { return oi(NP); }
+    static constexpr int npz() { return pzi(NP); }
+    // Rows of a phase (Z: the staged c_z != 0 populations of the arriving
+    // plane; !Z: the in-plane populations of plane k), in slot order.
+    // Staging instruction q of wave w gives lane group g (16 lanes of 4 x
+    // values each) row R = 4 (q BY + w) + g, clamped to the last row (a
+    // repeated row writes the same values).
     static constexpr int nrows(bool z) {
         int c = 0;
         for (int p = 0; p < NP; p++)
@@ -83,9 +113,8 @@ template <int BY> struct ZmLayout {
 
 // A thread's staging plan for one phase, fixed for the whole march: for
 // each staging instruction, the element offset of its four values at plane
-// 0 and their LDS index in ring slot 0 (low 16 bits) with the region size
-// (high 16 bits, the ring stride); the same for its halo value, if any
-// (thread t owns halo value t: population, region row, side).
+// 0 and their LDS index; the same for its halo value, if any (thread t owns
+// halo value t: population, region row, side).
 template <int BY, bool Z> struct ZmPlan {
     using L = ZmLayout<BY>;
     uint32_t goff[L::ninstr(Z)];
@@ -115,8 +144,7 @@ __device__ __forceinline__ ZmPlan<BY, Z> zm_plan(const StepArgs<float> &a, int i
                     const int rr = R - cum;
                     const int jj = min(max(j0 - L::hy(p) + rr, 0), n - 1);
                     go = (uint32_t)p * plane + (uint32_t)jj * n + ii;
-                    ld = (uint32_t)(L::off(p) + rr * L::P(p) + L::C0(p) + c4) |
-                         ((uint32_t)L::psize(p) << 16);
+                    ld = (uint32_t)(L::off(p) + rr * L::P(p) + L::C0(p) + c4);
                 }
                 cum += L::H(p);
             }
@@ -137,8 +165,7 @@ __device__ __forceinline__ ZmPlan<BY, Z> zm_plan(const StepArgs<float> &a, int i
             const bool inside = jj >= 0 && jj < n && ih >= 0 && ih < n;
             if (mine && inside) {
                 pl.hoff = (uint32_t)p * plane + (uint32_t)jj * n + ih;
-                pl.hlds = (uint32_t)(L::off(p) + rr * L::P(p) + (side ? 68 : 3)) |
-                          ((uint32_t)L::psize(p) << 16);
+                pl.hlds = (uint32_t)(L::off(p) + rr * L::P(p) + (side ? 68 : 3));
             }
             h -= cnt;
         }
@@ -170,93 +197,165 @@ __device__ __forceinline__ void zm_fetch(const StepArgs<float> &a, int kk,
     s.halo = a.pin[po + pl.hoff];
 }
 
-// Writes fetched values into their LDS slot (Z: ring slot of the plane),
-// as two 8-byte writes per group of four (fewer bank conflicts than one
-// 16-byte write).
+// The (0, 0, +-1) populations of the lane's own column at plane kk (clamped).
+template <int BY>
+__device__ __forceinline__ void zm_fetch_purez(const StepArgs<float> &a, uint32_t col, int kk,
+                                               float *v) {
+    using L = ZmLayout<BY>;
+    kk = min(kk, a.n - 1);
+    const uint32_t node = col + (uint32_t)kk * (uint32_t)a.n * (uint32_t)a.n;
+    static_for<0, L::NP>([&](auto Pp) {
+        constexpr int p = decltype(Pp)::value;
+        if constexpr (L::purez(p)) v[L::pzi(p)] = ldp(a.pin + p * a.plane, node);
+    });
+}
+
+// Writes fetched values into their LDS region as two 8-byte writes per
+// group of four (fewer bank conflicts than one 16-byte write).
 template <int BY, bool Z>
 __device__ __forceinline__ void zm_commit(float *lds, const ZmPlan<BY, Z> &pl,
-                                          const ZmStage<BY, Z> &s, int slot) {
+                                          const ZmStage<BY, Z> &s) {
     using L = ZmLayout<BY>;
     static_for<0, L::ninstr(Z)>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
-        const uint32_t idx = (pl.lds[q] & 0xffffu) + (Z ? (uint32_t)slot * (pl.lds[q] >> 16) : 0u);
-        *reinterpret_cast<zm_f2 *>(&lds[idx]) = s.quad[q].xy;
-        *reinterpret_cast<zm_f2 *>(&lds[idx + 2]) = s.quad[q].zw;
+        *reinterpret_cast<zm_f2 *>(&lds[pl.lds[q]]) = s.quad[q].xy;
+        *reinterpret_cast<zm_f2 *>(&lds[pl.lds[q] + 2]) = s.quad[q].zw;
     });
-    if (pl.hlds != ~0u)
-        lds[(pl.hlds & 0xffffu) + (Z ? (uint32_t)slot * (pl.hlds >> 16) : 0u)] = s.halo;
+    if (pl.hlds != ~0u) lds[pl.hlds] = s.halo;
 }
 
-// Value of slot p at x, y offsets (dx, dy) from the lane's node, in ring
-// slot zs (0 for populations staged on one plane).
+// Value of slot p at x, y offsets (dx, dy) from the lane's node.
 template <int BY, int p>
-__device__ __forceinline__ float zm_at(const float *lds, int l, int r, int dx, int dy,
-                                       int zs) {
+__device__ __forceinline__ float zm_at(const float *lds, int l, int r, int dx, int dy) {
     using L = ZmLayout<BY>;
-    return lds[L::off(p) + L::psize(p) * zs + (r + L::hy(p) + dy) * L::P(p) + L::C0(p) + l +
-               dx];
+    return lds[L::off(p) + (r + L::hy(p) + dy) * L::P(p) + L::C0(p) + l + dx];
 }
 
 // Value of slot p at stencil point S1 along axis A1 and S2 along axis A2
-// (point 1 is the node itself; A < 0 means no axis).  zsl[s] is the ring
-// slot of the plane of z stencil point s.
+// (in-plane axes; point 1 is the node itself; A < 0 means no axis).
 template <int BY, int p, int A1, int S1, int A2, int S2>
 __device__ __forceinline__ float zm_pt(const float *lds, int l, int r,
-                                       const AxisStencil<float> *st, const int *zsl) {
-    using L = ZmLayout<BY>;
-    int dx = 0, dy = 0, zs = zsl[1];
+                                       const AxisStencil<float> *st) {
+    int dx = 0, dy = 0;
     if constexpr (A1 == 0 && S1 != 1) dx = S1 == 0 ? st[0].o0 : st[0].o2;
     if constexpr (A2 == 0 && S2 != 1) dx = S2 == 0 ? st[0].o0 : st[0].o2;
     if constexpr (A1 == 1 && S1 != 1) dy = S1 == 0 ? st[1].o0 : st[1].o2;
     if constexpr (A2 == 1 && S2 != 1) dy = S2 == 0 ? st[1].o0 : st[1].o2;
-    if constexpr (A1 == 2 && S1 != 1) zs = zsl[S1];
-    if constexpr (A2 == 2 && S2 != 1) zs = zsl[S2];
-    return zm_at<BY, p>(lds, l, r, dx, dy, L::zpop(p) ? zs : 0);
+    return zm_at<BY, p>(lds, l, r, dx, dy);
 }
 
-// interp_pop with its operands from LDS: the same expressions in the same
-// order.
+// interp_pop of a c_z = 0 population with its operands from LDS: the same
+// expressions in the same order.
 template <int BY, int p>
-__device__ __forceinline__ float zm_interp(const float *lds, int l, int r, float fc,
-                                           const AxisStencil<float> *st, const int *zsl) {
+__device__ __forceinline__ float zm_interp_plane(const float *lds, int l, int r, float fc,
+                                                 const AxisStencil<float> *st) {
     using L = ZmLayout<BY>;
-    constexpr int c[3] = {L::cx(p), L::cy(p), L::cz(p)};
-    constexpr int nz = (c[0] != 0) + (c[1] != 0) + (c[2] != 0);
-    if constexpr (nz == 1) {
-        constexpr int ax = c[0] ? 0 : (c[1] ? 1 : 2);
+    constexpr int c[2] = {L::cx(p), L::cy(p)};
+    if constexpr (c[0] == 0 || c[1] == 0) {
+        constexpr int ax = c[0] ? 0 : 1;
         const float *w = c[ax] > 0 ? st[ax].wp : st[ax].wm;
-        return lerp3<false>(w, zm_pt<BY, p, ax, 0, -1, 1>(lds, l, r, st, zsl), fc,
-                            zm_pt<BY, p, ax, 2, -1, 1>(lds, l, r, st, zsl));
+        return lerp3<false>(w, zm_pt<BY, p, ax, 0, -1, 1>(lds, l, r, st), fc,
+                            zm_pt<BY, p, ax, 2, -1, 1>(lds, l, r, st));
     } else {
-        constexpr int q = c[0] ? 0 : 1;   // outer stage axis
-        constexpr int pp = c[2] ? 2 : 1;  // inner stage axis
-        const float *wpp = c[pp] > 0 ? st[pp].wp : st[pp].wm;
-        const float *wqq = c[q] > 0 ? st[q].wp : st[q].wm;
-        const float c0 = lerp3<false>(wpp, zm_pt<BY, p, q, 0, pp, 0>(lds, l, r, st, zsl),
-                                      zm_pt<BY, p, q, 0, pp, 1>(lds, l, r, st, zsl),
-                                      zm_pt<BY, p, q, 0, pp, 2>(lds, l, r, st, zsl));
-        const float c1 = lerp3<false>(wpp, zm_pt<BY, p, q, 1, pp, 0>(lds, l, r, st, zsl), fc,
-                                      zm_pt<BY, p, q, 1, pp, 2>(lds, l, r, st, zsl));
-        const float c2 = lerp3<false>(wpp, zm_pt<BY, p, q, 2, pp, 0>(lds, l, r, st, zsl),
-                                      zm_pt<BY, p, q, 2, pp, 1>(lds, l, r, st, zsl),
-                                      zm_pt<BY, p, q, 2, pp, 2>(lds, l, r, st, zsl));
-        return lerp3<false>(wqq, c0, c1, c2);
+        // outer stage along x, inner along y
+        const float *wy = c[1] > 0 ? st[1].wp : st[1].wm;
+        const float *wx = c[0] > 0 ? st[0].wp : st[0].wm;
+        const float c0 = lerp3<false>(wy, zm_pt<BY, p, 0, 0, 1, 0>(lds, l, r, st),
+                                      zm_pt<BY, p, 0, 0, 1, 1>(lds, l, r, st),
+                                      zm_pt<BY, p, 0, 0, 1, 2>(lds, l, r, st));
+        const float c1 = lerp3<false>(wy, zm_pt<BY, p, 0, 1, 1, 0>(lds, l, r, st), fc,
+                                      zm_pt<BY, p, 0, 1, 1, 2>(lds, l, r, st));
+        const float c2 = lerp3<false>(wy, zm_pt<BY, p, 0, 2, 1, 0>(lds, l, r, st),
+                                      zm_pt<BY, p, 0, 2, 1, 1>(lds, l, r, st),
+                                      zm_pt<BY, p, 0, 2, 1, 2>(lds, l, r, st));
+        return lerp3<false>(wx, c0, c1, c2);
     }
+}
+
+// Puts the arriving plane into ring entry S: for each c_z != 0 population
+// its in-plane interpolation at the lane's (i, j) and, for the staged ones,
+// its own value (the (0, 0, +-1) values come from pz).
+template <int BY, int S, int NZ, int NZO>
+__device__ __forceinline__ void zm_ring_in(const float *lds, int l, int r,
+                                           const AxisStencil<float> *st, const float *pz,
+                                           float (&zr)[NZ][3], float (&zo)[NZO][3]) {
+    using L = ZmLayout<BY>;
+    static_for<0, L::NP>([&](auto Pp) {
+        constexpr int p = decltype(Pp)::value;
+        if constexpr (L::purez(p)) {
+            zr[L::zi(p)][S] = pz[L::pzi(p)];
+        } else if constexpr (L::zpop(p)) {
+            constexpr int ax = L::cx(p) ? 0 : 1;
+            constexpr int cc = ax == 0 ? L::cx(p) : L::cy(p);
+            const float *w = cc > 0 ? st[ax].wp : st[ax].wm;
+            const float own = zm_at<BY, p>(lds, l, r, 0, 0);
+            zo[L::oi(p)][S] = own;
+            zr[L::zi(p)][S] = lerp3<false>(w, zm_pt<BY, p, ax, 0, -1, 1>(lds, l, r, st), own,
+                                           zm_pt<BY, p, ax, 2, -1, 1>(lds, l, r, st));
+        }
+    });
+}
+
+// Ring move to the next plane: entry 0 <- 1, 1 <- 2 (entry 2 is then
+// refilled from the arriving plane).
+template <int NZ, int NZO>
+__device__ __forceinline__ void zm_ring_shift(float (&zr)[NZ][3], float (&zo)[NZO][3]) {
+    static_for<0, NZ>([&](auto Z) {
+        constexpr int z = decltype(Z)::value;
+        zr[z][0] = zr[z][1];
+        zr[z][1] = zr[z][2];
+    });
+    static_for<0, NZO>([&](auto Z) {
+        constexpr int z = decltype(Z)::value;
+        zo[z][0] = zo[z][1];
+        zo[z][1] = zo[z][2];
+    });
 }
 
 __device__ __forceinline__ int zm_window(int k, int n) { return min(max(k - 1, 0), n - 3); }
 
+// Tile (tx, ty) and z group tz of hardware workgroup P; false for the
+// workgroups past the last z group.  remap 2 (gy >= 8): workgroups are
+// dispatched round-robin over the 8 XCDs, and XCD x = P % 8 owns the tile
+// rows [x gy / 8, (x + 1) gy / 8), swept x fastest, then y, then z, so the
+// x and y neighbour tiles that re-read a tile's rows run on the same XCD at
+// the same time and find them in its L2.  The grid gives every XCD the
+// largest share; the surplus workgroups of the smaller bands return at once.
+// Other remaps: the x-fastest order of logical_tile.
+__device__ __forceinline__ bool zm_tile(int P, const StepArgs<float> &a, int nzc, int &tx,
+                                        int &ty, int &tz) {
+    if (a.remap == 2) {
+        const int x = P & 7, q = P >> 3;
+        const int b0 = x * a.gy / 8, band = (x + 1) * a.gy / 8 - b0;
+        tx = q % a.gx;
+        const int rr = q / a.gx;
+        ty = b0 + rr % band;
+        tz = rr / band;
+    } else {
+        const int L = logical_tile(P, a.ntiles, a.remap);
+        tx = L % a.gx;
+        const int rr = L / a.gx;
+        ty = rr % a.gy;
+        tz = rr / a.gy;
+    }
+    return tz < nzc;
+}
+
 template <int BY, bool MACRO, bool NT>
-__global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) {
+#ifndef LBM_ZM_WAVES
+#define LBM_ZM_WAVES 1
+#endif
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(LBM_ZM_WAVES))) void k_step_zm(StepArgs<float> a, int kc) {
     using L = ZmLayout<BY>;
     using FL = Flow<3>;
     using HT = Heat<3>;
     constexpr int QF = FL::Q, QG = HT::Q;
+    constexpr int NZ = L::nz(), NZO = L::nzo();
     __shared__ zm_f4 lds4[L::total() / 4];
     float *lds = reinterpret_cast<float *>(lds4);
     const int n = a.n;
     int tx, ty, tc;
-    tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tc);
+    if (!zm_tile((int)blockIdx.x, a, (n + kc - 1) / kc, tx, ty, tc)) return;
     const int i0 = tx * 64, j0 = ty * BY;
     const int k0 = tc * kc, k1 = min(n, k0 + kc);
     const int l = threadIdx.x;
@@ -265,55 +364,80 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) 
     // lanes past the grid edge stage and compute like the last node and
     // store nothing (every lane must reach the barriers)
     const int i = min(i0 + l, n - 1), j = min(j0 + r, n - 1);
+    const uint32_t sy = n, sz2 = (uint32_t)n * n;
+    const uint32_t col = i + sy * j;
     AxisStencil<float> st[3];
     st[0] = load_stencil(a, i);
     st[1] = load_stencil(a, j);
     const ZmPlan<BY, false> pin_ = zm_plan<BY, false>(a, i0, j0, r, l);
     const ZmPlan<BY, true> pz = zm_plan<BY, true>(a, i0, j0, r, l);
 
-    {   // prologue: the z window of plane k0 and the in-plane rows of k0
+    // z ring of plane k0: the planes of its window, in order
+    float zr[NZ][3], zo[NZO][3];
+    {
         const int ws = zm_window(k0, n);
-        for (int zz = ws; zz < ws + 3; zz++) {
+        static_for<0, 3>([&](auto S) {
+            constexpr int s = decltype(S)::value;
             ZmStage<BY, true> sz;
-            zm_fetch<BY, true>(a, zz, pz, sz);
-            zm_commit<BY, true>(lds, pz, sz, zz % 3);
-        }
+            float pzv[L::npz()];
+            zm_fetch<BY, true>(a, ws + s, pz, sz);
+            zm_fetch_purez<BY>(a, col, ws + s, pzv);
+            zm_commit<BY, true>(lds, pz, sz);
+            __syncthreads();
+            zm_ring_in<BY, s>(lds, l, r, st, pzv, zr, zo);
+            __syncthreads();
+        });
         ZmStage<BY, false> sp;
         zm_fetch<BY, false>(a, k0, pin_, sp);
-        zm_commit<BY, false>(lds, pin_, sp, 0);
+        zm_commit<BY, false>(lds, pin_, sp);
     }
     __syncthreads();
 
-    const uint32_t sy = n, sz2 = (uint32_t)n * n;
     // constant along z (the heated axis is y)
     const HeatShift<float> hs = load_heat_shift<3, true>(a, {i, j, 0});
-    for (int k = k0; k < k1; k++) {
+
+    // One plane.  WALL: -1 for k = 0 (stencil planes 1, 0, 2 = ring entries
+    // 1, 0, 2), +1 for k = n-1 (n-2, n-1, n-3 = entries 1, 2, 0), 0 inside
+    // (entries 0, 1, 2).  SLIDE: the window moves to the next plane, whose
+    // arriving plane is fetched and interpolated (always inside a z group
+    // below n-2; never at the walls or on a group's last plane).
+    auto plane = [&](auto WALL, auto SLIDE, int k) {
+        constexpr int W = decltype(WALL)::value;
+        constexpr bool SL = decltype(SLIDE)::value;
+        constexpr int E0 = W == 0 ? 0 : (W < 0 ? 1 : 1);
+        constexpr int E1 = W == 0 ? 1 : (W < 0 ? 0 : 2);
+        constexpr int E2 = W == 0 ? 2 : (W < 0 ? 2 : 0);
         const int kn = k + 1;
         const bool more = kn < k1;
-        const int wsn = zm_window(kn, n);
-        const bool newz = more && wsn != zm_window(k, n);
         st[2] = load_stencil(a, k);
-        const int zsl[3] = {(k + st[2].o0) % 3, k % 3, (k + st[2].o2) % 3};
-        const uint32_t node = i + sy * j + sz2 * k;
+        const uint32_t node = col + sz2 * k;
         const int idx[3] = {i, j, k};
         float pcf[QF], pcg[QG], f[QF], g[QG];
-        // This plane's own loads go out before the next plane's rows: vector
+        // This plane's own loads go out before the next planes' rows: vector
         // loads complete in order, so the compute below waits only for them
-        // and the prefetch stays in flight behind it.  The prefetch is issued
-        // unconditionally (clamped planes) for the same reason.
+        // and the prefetch stays in flight behind it.  The in-plane prefetch
+        // is issued unconditionally (clamped plane) for the same reason.
         pcf[0] = ldp(a.pin, node);
         pcg[0] = ldp(a.pin + QF * a.plane, node);
         __builtin_amdgcn_sched_barrier(0);
         ZmStage<BY, false> np;
         ZmStage<BY, true> nzs;
+        float pzn[L::npz()];
         zm_fetch<BY, false>(a, kn, pin_, np);
-        zm_fetch<BY, true>(a, wsn + 2, pz, nzs);
+        if constexpr (SL) {
+            const int znew = k + 2;   // the plane that enters the window
+            zm_fetch<BY, true>(a, znew, pz, nzs);
+            zm_fetch_purez<BY>(a, col, znew, pzn);
+        }
         __builtin_amdgcn_sched_barrier(0);
 
         static_for<1, L::NP>([&](auto Pp) {
             constexpr int p = decltype(Pp)::value;
-            if constexpr (L::staged(p)) {
-                const float v = zm_at<BY, p>(lds, l, r, 0, 0, L::zpop(p) ? k % 3 : 0);
+            if constexpr (p != QF) {
+                float v;
+                if constexpr (L::purez(p)) v = zr[L::zi(p)][E1];
+                else if constexpr (L::zpop(p)) v = zo[L::oi(p)][E1];
+                else v = zm_at<BY, p>(lds, l, r, 0, 0);
                 if constexpr (p < QF) pcf[p] = v;
                 else pcg[p - QF] = v;
             }
@@ -323,23 +447,52 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) 
         static_for<0, QG>([&](auto A) { sp_g += pcg[decltype(A)::value]; });
         f[0] = pcf[0];
         g[0] = pcg[0];
-        static_for<1, QF>([&](auto A) {
-            constexpr int al = decltype(A)::value;
-            f[al] = zm_interp<BY, al>(lds, l, r, pcf[al], st, zsl);
-        });
-        static_for<1, QG>([&](auto A) {
-            constexpr int al = decltype(A)::value;
-            g[al] = zm_interp<BY, QF + al>(lds, l, r, pcg[al], st, zsl);
+        static_for<1, L::NP>([&](auto Pp) {
+            constexpr int p = decltype(Pp)::value;
+            if constexpr (p != QF) {
+                float v;
+                if constexpr (L::zpop(p)) {
+                    constexpr int z = L::zi(p);
+                    const float *w = L::cz(p) > 0 ? st[2].wp : st[2].wm;
+                    v = lerp3<false>(w, zr[z][E0], zr[z][E1], zr[z][E2]);
+                } else if constexpr (p < QF) {
+                    v = zm_interp_plane<BY, p>(lds, l, r, pcf[p], st);
+                } else {
+                    v = zm_interp_plane<BY, p>(lds, l, r, pcg[p - QF], st);
+                }
+                if constexpr (p < QF) f[p] = v;
+                else g[p - QF] = v;
+            }
         });
         if (live)
             node_tail<3, float, float, true, MACRO, NT>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
                                                         hs);
 
-        __syncthreads();   // every read of plane k's staging is done
-        if (more) zm_commit<BY, false>(lds, pin_, np, 0);
-        if (newz) zm_commit<BY, true>(lds, pz, nzs, (wsn + 2) % 3);
+        __syncthreads();   // every read of this plane's staging is done
+        if (more) zm_commit<BY, false>(lds, pin_, np);
+        if constexpr (SL) zm_commit<BY, true>(lds, pz, nzs);
         __syncthreads();
+        if constexpr (SL) {
+            zm_ring_shift(zr, zo);
+            zm_ring_in<BY, 2>(lds, l, r, st, pzn, zr, zo);
+        }
+    };
+
+    using std::integral_constant;
+    int k = k0;
+    if (k == 0) {
+        plane(integral_constant<int, -1>{}, std::false_type{}, 0);
+        k = 1;
     }
+    // planes whose next plane moves the window: k + 1 < k1 and k + 1 <= n - 2
+    const int kslide = min(k1 - 1, n - 2);
+    for (; k < kslide; k++) plane(integral_constant<int, 0>{}, std::true_type{}, k);
+    // an inner plane without a move: the group's last plane, or n - 2
+    if (k < min(k1, n - 1)) {
+        plane(integral_constant<int, 0>{}, std::false_type{}, k);
+        k++;
+    }
+    if (k == n - 1 && k < k1) plane(integral_constant<int, 1>{}, std::false_type{}, k);
 }
 
 }  // namespace lbm

@@ -262,16 +262,21 @@ def test_tile_orders_give_identical_results(hip, dim, n, bx, by):
     (96, 4, 0),
     (132, 4, 2),    # y bands per XCD
     (24, 8, 0),
-    (136, 8, 2),
+    (136, 8, 2),    # 17 tile rows: XCD bands of 2 and 3 rows
+    (24, 6, 0),     # far wall at tile row 5
+    (68, 6, 2),     # partial last tile row (2 of 6 rows), 12 tile rows in bands
+    (132, 6, 2),
+    (72, 4, 2),     # 18 tile rows in bands of 2 and 3
 ])
 def test_lds_zmarch_variant_gives_identical_results(hip, n, by, remap):
-    """Variant 1 stages the populations in LDS and marches in z; its
-    interpolation reads the same values in the same order as variant 0.  The
-    compiler fuses a different handful of multiply-adds in the two kernels,
-    so they agree to FP32 rounding, not to the bit: ~1e-7 of the velocity
-    scale after 24 steps, while a wrong stencil point or plane shows as an
-    error near 1e-2.  Bound: 1e-6 relative to each field's largest value,
-    walls and partial tiles included; accuracy against the oracle is checked
+    """Variant 1 stages the populations in LDS and marches in z; it uses the
+    same stencil values and weights as variant 0, but interpolates the c_z
+    != 0 populations in-plane first (as each plane arrives) and then along
+    z, and the compiler fuses a different handful of multiply-adds.  So the
+    two agree to FP32 rounding, not to the bit: ~1e-7 of the velocity scale
+    after 24 steps, while a wrong stencil point or plane shows as an error
+    near 1e-2.  Bound: 1e-6 relative to each field's largest value, walls and
+    partial tiles included; accuracy against the oracle is checked
     separately."""
     print(f"\n[z-march] n={n} by={by} remap={remap}", flush=True)
     outs = []
@@ -291,12 +296,12 @@ def test_lds_zmarch_variant_gives_identical_results(hip, n, by, remap):
         assert err <= 1e-6, (name, err)
 
 
-@pytest.mark.parametrize("n,steps", [(16, 300), (36, 200)])
-def test_lds_zmarch_variant_matches_oracle(oracle_mod, hip, n, steps):
+@pytest.mark.parametrize("n,steps,by", [(16, 300, 2), (36, 200, 4), (40, 200, 6)])
+def test_lds_zmarch_variant_matches_oracle(oracle_mod, hip, n, steps, by):
     """Variant 1 against the FP64 oracle, to the FP32 bar of the plain path."""
-    print(f"\n[z-march vs oracle] n={n} steps={steps}", flush=True)
+    print(f"\n[z-march vs oracle] n={n} steps={steps} by={by}", flush=True)
     s = hip.Solver(3, n, 1e5)
-    s.set_launch(64, 2, remap=0, variant=1)
+    s.set_launch(64, by, remap=0, variant=1)
     o = oracle_mod.Oracle(3, n, 1e5)
     s.init_rest()
     o.init_rest()
@@ -317,7 +322,8 @@ def test_lds_zmarch_variant_is_rejected_where_it_cannot_run(hip):
     cases = [(dict(dim=2, n=64), (64, 4)),            # 3D only
              (dict(dim=3, n=64, precision=hip.PREC_FP64), (64, 4)),
              (dict(dim=3, n=64), (128, 2)),          # bx must be 64
-             (dict(dim=3, n=64), (64, 16)),          # by 2, 4 or 8
+             (dict(dim=3, n=64), (64, 16)),          # by 2, 4, 6 or 8
+             (dict(dim=3, n=64), (64, 3)),
              (dict(dim=3, n=66), (64, 4)),           # rows not 16-byte aligned
              (dict(dim=3, n=65), (64, 2))]
     for kw, (bx, by) in cases:
