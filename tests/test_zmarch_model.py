@@ -1,0 +1,298 @@
+"""A CPU model of k_step_zm's indexing (csrc/lbm_step_zm.h) -- CPU only.
+
+The kernel stages rows in LDS from per-thread plans, keeps the c_z != 0
+populations in a three-entry register ring and peels the two wall planes
+into their own copies of the plane body.  None of that can fault or go
+stale without showing up somewhere in the GPU parity tests, but those only
+see the end result.  This model restates the layout, the staging plans, the
+workgroup-to-tile map and the march in Python and checks, for the launch
+shapes the GPU tests use:
+  * every LDS index written or read lies inside its population's region,
+    and every global element loaded lies inside the population planes;
+  * every value a lane reads (from LDS or from the ring) is the one the
+    interpolation needs: the right population, plane, row and column;
+  * the tile map hands every (x tile, y tile, z group) to exactly one
+    workgroup, and the march visits every plane of a group once, with the
+    wall copies at k = 0 and k = n-1 and the ring shift exactly when the z
+    window moves.
+"""
+import itertools
+
+import pytest
+
+# D3Q19 velocities in the kernel's order (slots 0..18), then D3Q7 (19..25)
+C = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+     (1, 1, 0), (-1, 1, 0), (1, -1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1),
+     (-1, 0, -1), (0, 1, 1), (0, -1, 1), (0, 1, -1), (0, -1, -1)]
+NP = 26
+
+
+def vel(p):
+    return C[p if p < 19 else p - 19]
+
+
+def zpop(p):
+    return vel(p)[2] != 0
+
+
+def purez(p):
+    return zpop(p) and vel(p)[0] == 0 and vel(p)[1] == 0
+
+
+def staged(p):
+    return p not in (0, 19) and not purez(p)
+
+
+class Layout:
+    def __init__(self, by):
+        self.by = by
+        self.hx = {p: int(vel(p)[0] != 0) for p in range(NP)}
+        self.hy = {p: int(vel(p)[1] != 0) for p in range(NP)}
+        self.P = {p: 72 if self.hx[p] else 64 for p in range(NP)}
+        self.C0 = {p: 4 if self.hx[p] else 0 for p in range(NP)}
+        self.H = {p: by + 2 * self.hy[p] for p in range(NP)}
+        self.off, o = {}, 0
+        for p in range(NP):
+            self.off[p] = o
+            o += self.P[p] * self.H[p] if staged(p) else 0
+        self.total = o
+        self.rows = {z: [(p, rr) for p in range(NP) if staged(p) and zpop(p) == z
+                         for rr in range(self.H[p])] for z in (False, True)}
+
+    def ninstr(self, z):
+        return (len(self.rows[z]) + 4 * self.by - 1) // (4 * self.by)
+
+    def region(self, p):
+        return self.off[p], self.off[p] + self.P[p] * self.H[p]
+
+    def at(self, p, l, r, dx, dy):
+        row, col = r + self.hy[p] + dy, self.C0[p] + l + dx
+        assert 0 <= row < self.H[p], (p, r, dy)
+        assert (3 if self.hx[p] else 0) <= col <= (68 if self.hx[p] else 63), (p, l, dx)
+        return self.off[p] + row * self.P[p] + col
+
+
+def stencil(i, n):
+    """Offsets of stencil points 0 and 2 (point 1 is the node)."""
+    if i == 0:
+        return 1, 2
+    if i == n - 1:
+        return -1, -2
+    return -1, 1
+
+
+def window(k, n):
+    return min(max(k - 1, 0), n - 3)
+
+
+def planes_per_group(n, by):
+    cols = ((n + 63) // 64) * ((n + by - 1) // by)
+    nzc = max(1, min((2048 + cols - 1) // cols, max(1, n // 4)))
+    return (n + nzc - 1) // nzc
+
+
+def tile_map(n, by, remap):
+    """Workgroup -> (tx, ty, tz) of zm_tile, with the launcher's grid size."""
+    kc = planes_per_group(n, by)
+    nzc = (n + kc - 1) // kc
+    gx, gy = (n + 63) // 64, (n + by - 1) // by
+    remap = 0 if (remap == 2 and gy < 8) else remap
+    ntiles = 8 * gx * ((gy + 7) // 8) * nzc if remap == 2 else gx * gy * nzc
+    out = {}
+    for P in range(ntiles):
+        if remap == 2:
+            x, q = P & 7, P >> 3
+            b0 = x * gy // 8
+            band = (x + 1) * gy // 8 - b0
+            tx, rr = q % gx, q // gx
+            ty, tz = b0 + rr % band, rr // band
+        else:
+            L = P
+            if remap == 1:
+                g8 = ntiles >> 3
+                if P < (g8 << 3):
+                    L = (P & 7) * g8 + (P >> 3)
+            tx, rr = L % gx, L // gx
+            ty, tz = rr % gy, rr // gy
+        if tz < nzc:
+            assert (tx, ty, tz) not in out.values()
+            out[P] = (tx, ty, tz)
+    assert sorted(out.values()) == sorted(itertools.product(range(gx), range(gy), range(nzc)))
+    return kc, gx, gy, nzc
+
+
+KERNEL_ORDER = {0: (0, 1, 2), -1: (1, 0, 2), 1: (1, 2, 0)}
+
+
+def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
+    """Replays one workgroup's march; returns the number of checked reads.
+    order: ring entries read as stencil points (o0, node, o2) by the body
+    copy for k = 0 (-1), inside (0) and k = n-1 (+1).  Reads are checked for
+    the lanes at both tile edges, around the middle and around the last live
+    lane of a partial tile; the staging covers every lane."""
+    L = Layout(by)
+    i0, j0 = tx * 64, ty * by
+    k0, k1 = tz * kc, min(n, (tz + 1) * kc)
+    lds = {}
+    checked = 0
+    # halo ownership: every halo value of a phase has a thread
+    for z in (False, True):
+        nh = sum(2 * L.H[p] for p in range(NP) if staged(p) and zpop(p) == z and L.hx[p])
+        assert nh <= 64 * by
+
+    def stage(z, kk):
+        kk = min(kk, n - 1)
+        rl, covered = L.rows[z], set()
+        for w, l in itertools.product(range(by), range(64)):
+            g, c4 = l >> 4, 4 * (l & 15)
+            ii = min(i0 + c4, n - 4)
+            for q in range(L.ninstr(z)):
+                R = min(4 * (q * by + w) + g, len(rl) - 1)
+                covered.add(R)
+                p, rr = rl[R]
+                jj = min(max(j0 - L.hy[p] + rr, 0), n - 1)
+                base = L.off[p] + rr * L.P[p] + L.C0[p] + c4
+                lo, hi = L.region(p)
+                g_idx = p * n ** 3 + kk * n * n + jj * n + ii   # population planes of n^3
+                assert lo <= base and base + 3 < hi and 0 <= g_idx and g_idx + 3 < NP * n ** 3
+                for e in range(4):
+                    lds[base + e] = (p, kk, jj, ii + e)
+            h = l + 64 * w
+            for p in range(NP):
+                if not (staged(p) and zpop(p) == z and L.hx[p]):
+                    continue
+                cnt = 2 * L.H[p]
+                if 0 <= h < cnt:
+                    rr, side = h >> 1, h & 1
+                    jj, ih = j0 - L.hy[p] + rr, (i0 + 64 if side else i0 - 1)
+                    if 0 <= jj < n and 0 <= ih < n:
+                        idx = L.off[p] + rr * L.P[p] + (68 if side else 3)
+                        lds[idx] = (p, kk, jj, ih)
+                h -= cnt
+        assert covered == set(range(len(rl)))
+
+    def inplane(p, l, r, i, j, m):
+        """The in-plane stage of z-population p at plane m: its LDS reads."""
+        nonlocal checked
+        cx, cy, _ = vel(p)
+        ax_off = stencil(i, n) if cx else stencil(j, n)
+        for d in (0,) + ax_off:
+            dx, dy = (d, 0) if cx else (0, d)
+            got = lds.get(L.at(p, l, r, dx, dy))
+            assert got == (p, m, j + dy, i + dx), (p, m, i, j, dx, dy, got)
+            checked += 1
+
+    last = min(63, n - 1 - i0)
+    cols = {0, 1, 2, 3, 30, 31, 32, 33, 60, 61, 62, 63, last - 1, last, min(63, last + 1)}
+    lanes = [(l, r) for r in range(by) for l in sorted(c for c in cols if 0 <= c < 64)]
+    ring = {}   # (lane, p) -> [tag of entry 0, 1, 2]
+
+    def ring_in(entry, m):
+        for l, r in lanes:
+            live = i0 + l < n and j0 + r < n
+            i, j = min(i0 + l, n - 1), min(j0 + r, n - 1)
+            for p in range(NP):
+                if not zpop(p):
+                    continue
+                if purez(p):
+                    tag = (p, min(m, n - 1))
+                else:
+                    if live:
+                        inplane(p, l, r, i, j, m)
+                    tag = (p, m)
+                ring.setdefault((l, r), {}).setdefault(p, [None] * 3)[entry] = tag
+
+    ws = window(k0, n)
+    for s in range(3):
+        stage(True, ws + s)
+        ring_in(s, ws + s)
+    stage(False, k0)
+
+    visited = []
+
+    def plane(W, SL, k):
+        nonlocal checked
+        visited.append((W, SL, k))
+        assert W == (-1 if k == 0 else (1 if k == n - 1 else 0))
+        more = k + 1 < k1
+        assert SL == (more and window(k + 1, n) != window(k, n))
+        E = order[W]
+        zo = stencil(k, n)
+        want_planes = (k + zo[0], k, k + zo[1])
+        for l, r in lanes:
+            live = i0 + l < n and j0 + r < n
+            i, j = min(i0 + l, n - 1), min(j0 + r, n - 1)
+            xo, yo = stencil(i, n), stencil(j, n)
+            for p in range(NP):
+                if p in (0, 19):
+                    continue
+                if zpop(p):
+                    ent = ring[(l, r)][p]
+                    assert tuple(ent[e][1] for e in E) == want_planes, (p, k, ent)
+                    checked += 1
+                    continue
+                cx, cy, _ = vel(p)
+                for dx, dy in itertools.product((0,) + (xo if cx else ()), (0,) + (yo if cy else ())):
+                    idx = L.at(p, l, r, dx, dy)
+                    if live:
+                        assert lds.get(idx) == (p, k, j + dy, i + dx), (p, k, i, j, dx, dy)
+                        checked += 1
+        if more:
+            stage(False, k + 1)
+        if SL:
+            stage(True, k + 2)
+            for lr in ring:
+                for p in ring[lr]:
+                    e = ring[lr][p]
+                    ring[lr][p] = [e[1], e[2], None]
+            ring_in(2, k + 2)
+
+    # the kernel's loop structure
+    k = k0
+    if k == 0:
+        plane(-1, False, 0)
+        k = 1
+    kslide = min(k1 - 1, n - 2)
+    while k < kslide:
+        plane(0, True, k)
+        k += 1
+    if k < min(k1, n - 1):
+        plane(0, False, k)
+        k += 1
+    if k == n - 1 and k < k1:
+        plane(1, False, k)
+    assert [v[2] for v in visited] == list(range(k0, k1))
+    return checked
+
+
+SHAPES = [(12, 2, 0), (20, 4, 0), (68, 4, 1), (72, 2, 2), (96, 4, 0), (132, 4, 2), (24, 8, 0),
+          (136, 8, 2), (24, 6, 0), (68, 6, 2), (132, 6, 2), (72, 4, 2), (16, 2, 0), (36, 4, 0),
+          (40, 6, 0)]
+
+
+@pytest.mark.parametrize("n,by,remap", SHAPES)
+def test_zmarch_indexing_model(n, by, remap):
+    kc, gx, gy, nzc = tile_map(n, by, remap)
+    edge = lambda g: sorted({0, 1, g - 2, g - 1} & set(range(g)))
+    total = 0
+    for tx, ty, tz in itertools.product(edge(gx), edge(gy), edge(nzc)):
+        total += march(n, by, tx, ty, tz, kc)
+    assert total > 0
+
+
+def test_zmarch_model_catches_a_wrong_ring_order():
+    """The model is not vacuous: a body that read the wall planes' z points
+    in window order (no peeled copies) is caught at k = 0 and at k = n-1."""
+    n, by = 12, 2
+    kc, gx, gy, nzc = tile_map(n, by, 0)
+    flat = {0: (0, 1, 2), -1: (0, 1, 2), 1: (0, 1, 2)}
+    with pytest.raises(AssertionError):
+        march(n, by, 0, 0, 0, kc, order=flat)
+    with pytest.raises(AssertionError):
+        march(n, by, 0, 0, nzc - 1, kc, order=flat)
+    assert march(n, by, 0, 0, 0, kc) > 0
+
+
+def test_tile_map_covers_every_tile_once_on_the_bench_grid():
+    for by, remap in ((4, 2), (6, 2), (8, 2), (4, 0), (4, 1)):
+        tile_map(256, by, remap)
