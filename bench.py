@@ -28,7 +28,7 @@ WORKLOADS = {
     "2d1025": dict(dim=2, n=1025, Ra=1e6, label="cavity2d_1025_ra1e6_d2q9_d2q5"),
     "batch64": dict(dim=2, count=64, seed=20261203,
                     label="batch_64_cavities2d_n65-513_A1-2.5_Ra1e3-1e6_Pr0.7-7"),
-    "warm64": dict(dim=2, n=64, held_out=24, seed=7,
+    "warm64": dict(dim=2, n=64, held_out=100, seed=7,
                    label="warm_start_64x64_cavities_Ra1e3-1e6_Pr0.7-7_A1-2.5"),
 }
 
