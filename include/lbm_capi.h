@@ -174,16 +174,19 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes);
  *  remap: tile order.  Workgroups are dealt round-robin to the 8 XCDs, each
  *    with its own L2.  0 = x fastest, then y, then z; 1 = each XCD sweeps one
  *    contiguous range of that order; 2 = each XCD owns a band of tile rows in
- *    y and sweeps it plane by plane (needs the number of y tiles divisible by
- *    8, else 0 is used).
+ *    y and sweeps it plane by plane (variant 0: needs the number of y tiles
+ *    divisible by 8, else 0 is used; variant 1 in 3D: at least 8 y tiles; in
+ *    2D, variant 1 treats 2 as 1).
  *  occupancy (3D, FP32 storage): 0, or 5 or 6 resident waves per SIMD, which
  *    caps the registers per lane.
  *  variant: step-kernel variant.  0 = one node per thread, every stencil
- *    point read through the caches.  1 (3D, FP32 storage) = each workgroup
- *    marches a 64 x by column of nodes in z with the populations staged in
- *    LDS; needs bx = 64, by = 2, 4 or 8, occupancy 0, n % 4 == 0 and
- *    n <= 544.  The variants read the same stencil values and
- *    agree to FP32 rounding. */
+ *    point read through the caches.  1 (FP32 storage, occupancy 0) = each
+ *    workgroup marches a strip of 64 columns through the grid with the
+ *    populations staged in LDS and a register ring:
+ *      3D: a 64 x by column of nodes in z; bx = 64, by = 2, 4, 6 or 8,
+ *          n % 4 == 0 and n <= 544;
+ *      2D: one wave per 64 columns, in y; bx = 64, by = 1.
+ *    The variants read the same stencil values and agree to FP32 rounding. */
 typedef struct lbm_launch {
     int32_t bx, by, remap, occupancy, variant;
 } lbm_launch;
@@ -254,8 +257,9 @@ int32_t lbm_batch_active(lbm_batch b, int32_t *cavities, int64_t *nodes);
 int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms);
 
 /* Block shape of the batched launch (bx a multiple of 32, bx*by a multiple
- * of 64 and at most 1024; remap, occupancy and variant must be 0), and its
- * autotuner (restores every cavity's state). */
+ * of 64 and at most 1024; remap and occupancy must be 0; variant 0, or 1 for
+ * the 2D y-march with bx = 64, by = 1), and its autotuner, which tries both
+ * variants (restores every cavity's state). */
 int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *launch);
 int32_t lbm_batch_get_launch(lbm_batch b, lbm_launch *launch);
 int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms);

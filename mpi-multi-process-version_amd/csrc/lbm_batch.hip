@@ -32,6 +32,8 @@ struct lbm_batch_ctx {
     int parity = 0;   // launches since the table was built, mod 2
     bool dirty = true;
     int bx = 64, by = 4;
+    int variant = 0;   // 0: one node per thread; 1: y-march strips
+    int kc = 0;        // variant 1: rows per chunk
 };
 
 namespace {
@@ -73,7 +75,11 @@ bool table_stale(const lbm_batch b) {
     return false;
 }
 
-// Lists the tiles of every active cavity, largest cavity first.
+// Lists the tiles of every active cavity, largest cavity first.  Variant 1
+// lists (cavity, strip, chunk) entries and then deals them to the 8 XCDs in
+// contiguous runs (workgroup b runs on XCD b mod 8), so that the strips
+// whose edge columns a strip reads run on its XCD; the table is padded to
+// a multiple of 8 with empty entries.
 int32_t rebuild(lbm_batch b) {
     BATCH_TRY(hipStreamSynchronize(b->stream));   // the old table may still be read
     std::vector<int> order(b->cav.size());
@@ -81,13 +87,33 @@ int32_t rebuild(lbm_batch b) {
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return b->cav[x]->n > b->cav[y]->n; });
     std::vector<int4> tiles;
+    if (b->variant == 1) {
+        double strip_rows = 0.0;
+        for (int c : order)
+            if (b->active[c]) strip_rows += (double)((b->cav[c]->n + 63) / 64) * b->cav[c]->n;
+        b->kc = lbm::ym_rows_per_chunk(strip_rows);
+    }
     for (int c : order) {
         if (!b->active[c]) continue;
         const lbm_ctx &h = *b->cav[c];
         b->base_cur[c] = h.cur;
-        const int gx = (h.n + b->bx - 1) / b->bx, gy = (h.n + b->by - 1) / b->by;
-        for (int ty = 0; ty < gy; ty++)
-            for (int tx = 0; tx < gx; tx++) tiles.push_back(make_int4(c, tx, ty, 2 * c + h.cur));
+        if (b->variant == 1) {
+            const int gx = (h.n + 63) / 64, nch = (h.n + b->kc - 1) / b->kc;
+            for (int ch = 0; ch < nch; ch++)
+                for (int tx = 0; tx < gx; tx++)
+                    tiles.push_back(make_int4(c, tx, ch, 2 * c + h.cur));
+        } else {
+            const int gx = (h.n + b->bx - 1) / b->bx, gy = (h.n + b->by - 1) / b->by;
+            for (int ty = 0; ty < gy; ty++)
+                for (int tx = 0; tx < gx; tx++)
+                    tiles.push_back(make_int4(c, tx, ty, 2 * c + h.cur));
+        }
+    }
+    if (b->variant == 1 && !tiles.empty()) {
+        const size_t per = (tiles.size() + 7) / 8;
+        std::vector<int4> dealt(8 * per, make_int4(-1, 0, 0, 0));
+        for (size_t L = 0; L < tiles.size(); L++) dealt[(L % per) * 8 + L / per] = tiles[L];
+        tiles.swap(dealt);
     }
     if ((int)tiles.size() > b->tiles_cap) {
         if (b->d_tiles) BATCH_TRY(hipFree(b->d_tiles));
@@ -119,9 +145,13 @@ int32_t batch_steps(lbm_batch b, int64_t nsteps) {
         if (b->active[c]) nodes += (double)b->cav[c]->nn;
     const bool nt = lbm::stream_stores(nodes * 14 * 4.0 * 2.0);
     for (int64_t s = 0; s < nsteps; s++) {
-        hipError_t e = lbm::launch_step_batch_f32(s == nsteps - 1, b->d_args, b->d_tiles,
-                                                  b->ntiles, b->parity, b->bx, b->by, nt,
-                                                  b->stream);
+        const bool macro = s == nsteps - 1;
+        hipError_t e = b->variant == 1
+                           ? lbm::launch_step_batch_ym_f32(macro, b->d_args, b->d_tiles,
+                                                           b->ntiles, b->parity, b->kc, nt,
+                                                           b->stream)
+                           : lbm::launch_step_batch_f32(macro, b->d_args, b->d_tiles, b->ntiles,
+                                                        b->parity, b->bx, b->by, nt, b->stream);
         if (e != hipSuccess) return hip_err(e, "batched step launch");
         b->parity ^= 1;
     }
@@ -341,17 +371,20 @@ int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms) {
 int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *l) {
     if (!b || !l) return set_error(LBM_ERR_ARG, "null argument");
     if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
-        l->remap != 0 || l->occupancy != 0 || l->variant != 0)
-        return set_error(LBM_ERR_ARG, "invalid batched launch configuration");
+        l->remap != 0 || l->occupancy != 0 || l->variant < 0 || l->variant > 1 ||
+        (l->variant == 1 && (l->bx != 64 || l->by != 1)))
+        return set_error(LBM_ERR_ARG, "invalid batched launch configuration (variant 1: bx = 64, "
+                                      "by = 1)");
     b->bx = l->bx;
     b->by = l->by;
+    b->variant = l->variant;
     b->dirty = true;
     return LBM_OK;
 }
 
 int32_t lbm_batch_get_launch(lbm_batch b, lbm_launch *l) {
     if (!b || !l) return set_error(LBM_ERR_ARG, "null argument");
-    *l = lbm_launch{b->bx, b->by, 0, 0, 0};
+    *l = lbm_launch{b->bx, b->by, 0, 0, b->variant};
     return LBM_OK;
 }
 
@@ -369,14 +402,17 @@ int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms
         saved[c].resize((size_t)bytes);
         if ((rc = lbm_save_state(b->cav[c], saved[c].data(), bytes))) return rc;
     }
-    static const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {64, 8}, {128, 1},
-                                  {128, 2}, {128, 4}, {256, 1}, {256, 2}, {32, 8}};
-    const int bx0 = b->bx, by0 = b->by;
-    int best_bx = bx0, best_by = by0;
+    // (bx, by, variant); variant 1 is the y-march
+    static const int cand[][3] = {{64, 1, 0},  {64, 2, 0},  {64, 4, 0},  {64, 8, 0},
+                                  {128, 1, 0}, {128, 2, 0}, {128, 4, 0}, {256, 1, 0},
+                                  {256, 2, 0}, {32, 8, 0},  {64, 1, 1}};
+    const int bx0 = b->bx, by0 = b->by, v0 = b->variant;
+    int best_bx = bx0, best_by = by0, best_v = v0;
     double best = 1e300;
     for (const auto &cd : cand) {
         b->bx = cd[0];
         b->by = cd[1];
+        b->variant = cd[2];
         b->dirty = true;
         double ms = 0.0;
         if ((rc = batch_steps(b, 1)) || (rc = lbm_batch_time_steps(b, steps_per_trial, &ms)))
@@ -385,10 +421,12 @@ int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms
             best = ms / steps_per_trial;
             best_bx = cd[0];
             best_by = cd[1];
+            best_v = cd[2];
         }
     }
     b->bx = rc ? bx0 : best_bx;
     b->by = rc ? by0 : best_by;
+    b->variant = rc ? v0 : best_v;
     b->dirty = true;
     BATCH_TRY(hipStreamSynchronize(b->stream));
     for (size_t c = 0; c < nc; c++) {

@@ -637,9 +637,12 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes) {
 // sits in the first column or row of a tile and its one-sided stencil stays
 // inside the staged halo.  Its staging offsets are 32-bit element offsets
 // over the 26 planes (26 n^3 < 2^32: n <= 544).
+// Launch variant 1 in 2D (the y-march) loads per lane and has no alignment
+// condition: FP32 storage, bx = 64 and by = 1 (one wave per workgroup).
 static bool zm_launch_ok(const lbm_ctx &c, int bx, int by, int occ) {
-    return c.dim == 3 && !c.fp64 && bx == 64 && (by == 2 || by == 4 || by == 6 || by == 8) &&
-           occ == 0 && c.n % 4 == 0 && c.n <= 544;
+    if (c.fp64 || bx != 64 || occ != 0) return false;
+    if (c.dim == 2) return by == 1;
+    return (by == 2 || by == 4 || by == 6 || by == 8) && c.n % 4 == 0 && c.n <= 544;
 }
 
 int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
@@ -651,8 +654,8 @@ int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
         return fail(LBM_ERR_ARG, "invalid launch configuration");
     if (l->variant == 1 && !zm_launch_ok(*h, l->bx, l->by, l->occupancy))
         return fail(LBM_ERR_ARG,
-                    "variant 1 needs 3D FP32 storage, bx = 64, by = 2, 4, 6 or 8, occupancy 0, "
-                    "n % 4 == 0 and n <= 544");
+                    "variant 1 needs FP32 storage, bx = 64, occupancy 0 and, in 2D, by = 1; "
+                    "in 3D, by = 2, 4, 6 or 8, n % 4 == 0 and n <= 544");
     h->shape.bx = l->bx;
     h->shape.by = l->by;
     h->shape.remap = l->remap;

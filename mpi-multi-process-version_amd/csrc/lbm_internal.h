@@ -62,11 +62,12 @@ struct Ctx {
 void fill_args(Ctx &c);
 
 // Capacity of the memory-side last-level cache (MALL, Infinity Cache) of
-// MI355X.  A step whose population traffic (read + write) exceeds it runs
-// with streaming stores; below it, ordinary stores keep the written planes
-// in the MALL for the next step.
+// MI355X.  The next step reads what a step writes: while the written
+// populations (half of a step's read + write traffic) fit in the MALL,
+// ordinary stores leave them there for it; beyond that they only displace
+// rows that are still to be read, and the step stores with streaming stores.
 constexpr double kMallBytes = 256.0 * 1024 * 1024;
-inline bool stream_stores(double bytes_per_step) { return bytes_per_step > kMallBytes; }
+inline bool stream_stores(double bytes_per_step) { return 0.5 * bytes_per_step > kMallBytes; }
 
 // Launchers (lbm_step_f32.hip / lbm_step_f64.hip).
 hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a,
@@ -89,6 +90,12 @@ hipError_t launch_nusselt(const Ctx &c);    // results: d_red[4..5]
 // FP32 shifted storage.
 hipError_t launch_step_batch_f32(bool macro, const StepArgs<float> *args, const int4 *tiles,
                                  int ntiles, int parity, int bx, int by, bool nt, hipStream_t s);
+// The batch's variant 1: the y-march over a table of (cavity, strip, chunk,
+// argument index) entries, kc rows per chunk; cavity < 0 marks padding.
+hipError_t launch_step_batch_ym_f32(bool macro, const StepArgs<float> *args, const int4 *tiles,
+                                    int ntiles, int parity, int kc, bool nt, hipStream_t s);
+// Rows per chunk of the y-march for a total of strip_rows (strips x rows).
+int ym_rows_per_chunk(double strip_rows);
 
 constexpr int kRedThreads = 256;
 constexpr int kRedMaxBlocks = 1024;

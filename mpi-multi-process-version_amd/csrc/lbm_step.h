@@ -431,6 +431,17 @@ __device__ __forceinline__ R ldp(const R *__restrict__ base, uint32_t idx) {
     return *reinterpret_cast<const R *>(reinterpret_cast<const char *>(base) +
                                         idx * (uint32_t)sizeof(R));
 }
+// Workgroup barrier for LDS handoffs: orders LDS accesses only.  The march
+// kernels exchange data between threads through LDS alone (global loads go
+// to the loading thread's registers, and each thread stores its own nodes),
+// so the barrier need not wait for outstanding global loads or stores, as
+// __syncthreads does; the prefetch and the stores stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // NT: a streaming (non-temporal) store, for states larger than the MALL:
 // the written planes are not read again within the step, so they should not
 // displace the rows that neighbouring workgroups are about to re-read.

@@ -1,9 +1,11 @@
 // lbm_step_f32.hip -- FP32 shifted-population instantiation of the fused
 // step (the production path).  Built with default FMA contraction.
 #include "lbm_internal.h"
+#include "lbm_step_ym.h"
 #include "lbm_step_zm.h"
 
 #include <algorithm>
+#include <cmath>
 
 namespace lbm {
 
@@ -33,12 +35,13 @@ static double step_bytes_f32(int dim, int n) {
 hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
                           hipStream_t s);
 
+hipError_t launch_step_ym(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
+                          hipStream_t s);
+
 hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
                            hipStream_t s) {
-    if (ls.variant == 1) {
-        if (dim != 3) return hipErrorInvalidValue;
-        return launch_step_zm(macro, a0, ls, s);
-    }
+    if (ls.variant == 1)
+        return dim == 3 ? launch_step_zm(macro, a0, ls, s) : launch_step_ym(macro, a0, ls, s);
     StepArgs<float> a = a0;
     dim3 block(ls.bx, ls.by, 1);
     a.gx = (a.n + ls.bx - 1) / ls.bx;
@@ -92,6 +95,52 @@ hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchSha
     else if (ls.by == 6) launch_zm<6>(macro, nt, grid, block, a, kc, s);
     else if (ls.by == 8) launch_zm<8>(macro, nt, grid, block, a, kc, s);
     else launch_zm<4>(macro, nt, grid, block, a, kc, s);
+    return hipGetLastError();
+}
+
+// Launch variant 1 (2D): y-march, one wave per strip of 64 columns over
+// kc rows; kc is chosen for about 4096 one-wave workgroups (16 per CU), but
+// at least 8 rows, so that the three rows a chunk loads beyond its own
+// stay a small share of its traffic.
+int ym_rows_per_chunk(double strip_rows) {
+    const long kc = (long)std::ceil(strip_rows / 4096.0);
+    return (int)std::max(8L, kc);
+}
+
+hipError_t launch_step_ym(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
+                          hipStream_t s) {
+    if (ls.bx != 64 || ls.by != 1) return hipErrorInvalidValue;
+    StepArgs<float> a = a0;
+    a.gx = (a.n + 63) / 64;
+    const int kc = std::min(a.n, ym_rows_per_chunk((double)a.gx * a.n));
+    a.gy = (a.n + kc - 1) / kc;
+    a.ntiles = a.gx * a.gy;
+    a.remap = ls.remap == 0 ? 0 : 1;
+    dim3 block(64, 1, 1), grid(a.ntiles);
+    const bool nt = stream_stores(step_bytes_f32(2, a.n));
+    if (macro && nt) hipLaunchKernelGGL((k_step_ym<true, true>), grid, block, 0, s, a, kc);
+    else if (macro) hipLaunchKernelGGL((k_step_ym<true, false>), grid, block, 0, s, a, kc);
+    else if (nt) hipLaunchKernelGGL((k_step_ym<false, true>), grid, block, 0, s, a, kc);
+    else hipLaunchKernelGGL((k_step_ym<false, false>), grid, block, 0, s, a, kc);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_batch_ym_f32(bool macro, const StepArgs<float> *args, const int4 *tiles,
+                                    int ntiles, int parity, int kc, bool nt, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    dim3 block(64, 1, 1), grid(ntiles);
+    if (macro && nt)
+        hipLaunchKernelGGL((k_step_ym_batch<true, true>), grid, block, 0, s, args, tiles, parity,
+                           kc);
+    else if (macro)
+        hipLaunchKernelGGL((k_step_ym_batch<true, false>), grid, block, 0, s, args, tiles, parity,
+                           kc);
+    else if (nt)
+        hipLaunchKernelGGL((k_step_ym_batch<false, true>), grid, block, 0, s, args, tiles, parity,
+                           kc);
+    else
+        hipLaunchKernelGGL((k_step_ym_batch<false, false>), grid, block, 0, s, args, tiles,
+                           parity, kc);
     return hipGetLastError();
 }
 

@@ -18,7 +18,7 @@
 //     those of k_step, applied in-plane first;
 //   * populations with c = (0, 0, +-1) have no in-plane neighbours: each
 //     lane loads its own value of the arriving plane into the ring;
-//   * the two rest populations are read by each lane directly.
+//   * the two rest populations are loaded by each lane, one plane ahead.
 // The ring holds the planes of the window [ws, ws + 2], ws = clamp(k - 1,
 // 0, n - 3), in order: a step to the next plane shifts it and interpolates
 // the arriving plane, except at the walls, where the window does not move.
@@ -383,18 +383,21 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(LBM_ZM_
             zm_fetch<BY, true>(a, ws + s, pz, sz);
             zm_fetch_purez<BY>(a, col, ws + s, pzv);
             zm_commit<BY, true>(lds, pz, sz);
-            __syncthreads();
+            lds_barrier();
             zm_ring_in<BY, s>(lds, l, r, st, pzv, zr, zo);
-            __syncthreads();
+            lds_barrier();
         });
         ZmStage<BY, false> sp;
         zm_fetch<BY, false>(a, k0, pin_, sp);
         zm_commit<BY, false>(lds, pin_, sp);
     }
-    __syncthreads();
+    lds_barrier();
 
     // constant along z (the heated axis is y)
     const HeatShift<float> hs = load_heat_shift<3, true>(a, {i, j, 0});
+    // the rest populations of the next plane, loaded one plane ahead
+    float rest_f = ldp(a.pin, col + sz2 * k0);
+    float rest_g = ldp(a.pin + QF * a.plane, col + sz2 * k0);
 
     // One plane.  WALL: -1 for k = 0 (stencil planes 1, 0, 2 = ring entries
     // 1, 0, 2), +1 for k = n-1 (n-2, n-1, n-3 = entries 1, 2, 0), 0 inside
@@ -413,13 +416,18 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(LBM_ZM_
         const uint32_t node = col + sz2 * k;
         const int idx[3] = {i, j, k};
         float pcf[QF], pcg[QG], f[QF], g[QG];
-        // This plane's own loads go out before the next planes' rows: vector
-        // loads complete in order, so the compute below waits only for them
-        // and the prefetch stays in flight behind it.  The in-plane prefetch
-        // is issued unconditionally (clamped plane) for the same reason.
-        pcf[0] = ldp(a.pin, node);
-        pcg[0] = ldp(a.pin + QF * a.plane, node);
-        __builtin_amdgcn_sched_barrier(0);
+        // Every value this plane reads was loaded during the previous one, so
+        // the loads issued here (the next planes' rows and rest populations)
+        // stay in flight through the whole compute; the in-plane prefetch is
+        // issued unconditionally (clamped plane) so that the waits before the
+        // commit count a fixed number of loads.
+        pcf[0] = rest_f;
+        pcg[0] = rest_g;
+        {
+            const uint32_t nn = col + sz2 * (uint32_t)min(kn, n - 1);
+            rest_f = ldp(a.pin, nn);
+            rest_g = ldp(a.pin + QF * a.plane, nn);
+        }
         ZmStage<BY, false> np;
         ZmStage<BY, true> nzs;
         float pzn[L::npz()];
@@ -468,10 +476,10 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(LBM_ZM_
             node_tail<3, float, float, true, MACRO, NT>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
                                                         hs);
 
-        __syncthreads();   // every read of this plane's staging is done
+        lds_barrier();   // every read of this plane's staging is done
         if (more) zm_commit<BY, false>(lds, pin_, np);
         if constexpr (SL) zm_commit<BY, true>(lds, pz, nzs);
-        __syncthreads();
+        lds_barrier();
         if constexpr (SL) {
             zm_ring_shift(zr, zo);
             zm_ring_in<BY, 2>(lds, l, r, st, pzn, zr, zo);

@@ -375,8 +375,8 @@ class Batch:
         _check(self._L.lbm_batch_time_steps(self.h, int(nsteps), ctypes.byref(ms)))
         return ms.value
 
-    def set_launch(self, bx, by):
-        cfg = Launch(bx=bx, by=by, remap=0, occupancy=0, variant=0)
+    def set_launch(self, bx, by, variant=0):
+        cfg = Launch(bx=bx, by=by, remap=0, occupancy=0, variant=variant)
         _check(self._L.lbm_batch_set_launch(self.h, ctypes.byref(cfg)))
 
     def get_launch(self):

@@ -52,6 +52,30 @@ def test_batch_matches_single_cavities(hip):
     b.close()
 
 
+def test_batch_ymarch_matches_single_cavities_to_the_bit(hip):
+    """The batch's variant 1 runs the same y-march code as a single cavity's
+    variant 1, over a table dealt to the XCDs: every cavity must match its
+    single-cavity run bit for bit (the chunk split does not change any
+    node's arithmetic)."""
+    print("\n[batch y-march] 5 cavities vs single y-march solvers, 300 steps", flush=True)
+    cav = CAVITIES + [dict(n=129, Ra=1e6, Pr=0.71, stretch=2.0)]
+    b = hip.Batch(cav)
+    b.set_launch(64, 1, variant=1)
+    assert b.get_launch()["variant"] == 1
+    b.init_rest()
+    b.step(300)
+    for i, c in enumerate(cav):
+        s = _single(hip, c)
+        s.set_launch(64, 1, remap=1, variant=1)
+        s.init_rest()
+        s.step(300)
+        got, ref = b[i].fields(), s.fields()
+        for x, y in zip(got, ref):
+            assert np.array_equal(x, y), (i, c)
+        s.close()
+    b.close()
+
+
 def test_batch_matches_oracle(hip, oracle_mod):
     print("\n[batch] vs oracle, 2 cavities, 1000 steps", flush=True)
     cav = [CAVITIES[0], CAVITIES[1]]

@@ -318,8 +318,58 @@ def test_lds_zmarch_variant_matches_oracle(oracle_mod, hip, n, steps, by):
     o.close()
 
 
+@pytest.mark.parametrize("n,remap", [(17, 0), (64, 1), (65, 1), (66, 0), (129, 1), (130, 0),
+                                     (257, 1)])
+def test_ymarch_variant_gives_identical_results(hip, n, remap):
+    """The 2D y-march (variant 1: one wave per 64 columns, marching in y)
+    against variant 0 after 24 steps: the same stencil values and weights,
+    with the x stage of the diagonal populations first, so they agree to
+    FP32 rounding (bound 1e-6 of each field's largest value).  n = 65, 129,
+    257 put the far x wall on lane 0 of the last strip; n = 66, 130 on lane
+    1."""
+    print(f"\n[y-march] n={n} remap={remap}", flush=True)
+    outs = []
+    for variant, bx, by in ((0, 64, 4), (1, 64, 1)):
+        s = hip.Solver(2, n, 1e5)
+        s.set_launch(bx, by, remap=remap, variant=variant)
+        assert s.get_launch()["variant"] == variant
+        s.init_rest()
+        s.step(24)
+        outs.append(s.fields())
+        s.close()
+    vscale = max(np.abs(outs[0][k]).max() for k in (1, 2))
+    for name, x, y, scale in zip(("rho", "u", "v", "w", "T"), outs[0], outs[1],
+                                 (1.0, vscale, vscale, 1.0, np.abs(outs[0][4]).max())):
+        err = np.abs(x - y).max() / scale
+        print(f"  {name}: {err:.3g}", flush=True)
+        assert err <= 1e-6, (name, err)
+
+
+@pytest.mark.parametrize("n,steps", [(33, 600), (65, 400)])
+def test_ymarch_variant_matches_oracle(oracle_mod, hip, n, steps):
+    """The 2D y-march against the FP64 oracle, to the FP32 bar of the plain path."""
+    print(f"\n[y-march vs oracle] n={n} steps={steps}", flush=True)
+    s = hip.Solver(2, n, 1e5)
+    s.set_launch(64, 1, remap=1, variant=1)
+    o = oracle_mod.Oracle(2, n, 1e5)
+    s.init_rest()
+    o.init_rest()
+    s.step(steps)
+    o.step(steps)
+    errs = field_errors(s.fields(), o.fields())
+    print(f"  {errs}", flush=True)
+    for k in ("rho", "u", "v", "T"):
+        assert errs[k] <= 1e-5, (k, errs[k])
+    nh, nc = s.nusselt()
+    oh, oc = o.nusselt()
+    assert abs(nh - oh) <= 1e-4 * abs(oh) and abs(nc - oc) <= 1e-4 * abs(oc)
+    s.close()
+    o.close()
+
+
 def test_lds_zmarch_variant_is_rejected_where_it_cannot_run(hip):
-    cases = [(dict(dim=2, n=64), (64, 4)),            # 3D only
+    cases = [(dict(dim=2, n=64), (64, 4)),            # 2D: by = 1
+             (dict(dim=2, n=64, precision=hip.PREC_FP64), (64, 1)),
              (dict(dim=3, n=64, precision=hip.PREC_FP64), (64, 4)),
              (dict(dim=3, n=64), (128, 2)),          # bx must be 64
              (dict(dim=3, n=64), (64, 16)),          # by 2, 4, 6 or 8
