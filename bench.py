@@ -235,6 +235,8 @@ def main():
                     help="oracle steps for the CPU baseline (0: automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true",
+                    help="skip the FP64 GPU run that the FP32 fields and Nu are compared with")
     ap.add_argument("--launch", default="",
                     help="fixed launch shape bx,by,remap,occupancy,variant (implies no autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -291,7 +293,28 @@ def main():
     achieved = bpu * nodes / launch_s / 1e9
     eu, et = s.check(allow_diverged=True)
     nu = s.nusselt()
-    s.close()
+    vs_fp64 = None
+    if prec == lbmhip.PREC_FP32_SHIFTED and not args.no_fp64_check:
+        # the FP64 GPU path after the same number of steps from the same
+        # initial condition: field and Nusselt errors of the FP32 run
+        import numpy as np
+        itc = s.itc
+        f32 = s.fields()
+        s.close()
+        d = lbmhip.Solver(cfg["dim"], cfg["n"], cfg["Ra"], precision=lbmhip.PREC_FP64, device=0)
+        d.init_rest()
+        d.step(itc)
+        f64, nu64 = d.fields(), d.nusselt()
+        d.close()
+        vscale = max(np.abs(f64[k]).max() for k in range(1, 1 + cfg["dim"]))
+        vs_fp64 = {"steps": itc,
+                   "velocity_max_abs_err_rel_to_max_speed": float(max(
+                       np.abs(f32[k] - f64[k]).max() for k in range(1, 1 + cfg["dim"])) / vscale),
+                   "T_max_abs_err": float(np.abs(f32[4] - f64[4]).max()),
+                   "nu_hot_rel_err": abs(nu[0] - nu64[0]) / abs(nu64[0]),
+                   "nu_cold_rel_err": abs(nu[1] - nu64[1]) / abs(nu64[1])}
+    else:
+        s.close()
 
     traffic = traffic_from_profiles(args.workload, args.precision, launch)
     out = {
@@ -317,7 +340,8 @@ def main():
                      "traffic": traffic,
                      "algorithmic_bytes_per_launch": bpu * nodes},
         "wall_s": round(wall, 4),
-        "check": {"errU": eu, "errT": et, "nu_hot": nu[0], "nu_cold": nu[1]},
+        "check": {"errU": eu, "errT": et, "nu_hot": nu[0], "nu_cold": nu[1],
+                  "vs_fp64_gpu": vs_fp64},
     }
     if not args.no_cpu_baseline:
         steps = args.cpu_steps or (10 if cfg["dim"] == 3 else 300)
