@@ -162,6 +162,8 @@ void free_ctx(Ctx *c) {
                     c->d_part, c->d_red};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipGraphExec_t &g : c->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -214,7 +216,7 @@ int32_t check_handle(lbm_handle h, bool need_init) {
     return set_device(*h);
 }
 
-int32_t do_steps(Ctx &c, int64_t nsteps) {
+int32_t launch_steps(Ctx &c, int64_t nsteps) {
     for (int64_t s = 0; s < nsteps; s++) {
         const bool macro = (s == nsteps - 1);
         bind_buffers(c);
@@ -224,6 +226,58 @@ int32_t do_steps(Ctx &c, int64_t nsteps) {
         c.cur = 1 - c.cur;
         c.itc++;
     }
+    return LBM_OK;
+}
+
+// Grids up to this many nodes replay a check interval as a HIP graph: their
+// step kernel runs for a few microseconds, about what a launch costs.
+constexpr long kGraphMaxNodes = 1L << 20;
+
+bool same_shape(const lbm::LaunchShape &a, const lbm::LaunchShape &b) {
+    return a.bx == b.bx && a.by == b.by && a.remap == b.remap && a.occ == b.occ &&
+           a.variant == b.variant;
+}
+
+// nsteps steps on the context's stream (asynchronous).  A check interval of
+// a small grid goes through a graph captured from the same launches (the
+// graph holds only kernel launches; each buffer parity has its own graph);
+// any other count is launched directly.
+int32_t do_steps(Ctx &c, int64_t nsteps) {
+    if (nsteps < 2 || nsteps != c.p.check_every || c.nn > kGraphMaxNodes)
+        return launch_steps(c, nsteps);
+    if (c.gsteps != nsteps || !same_shape(c.gshape, c.shape)) {
+        for (hipGraphExec_t &g : c.graph)
+            if (g) {
+                HIP_TRY(hipGraphExecDestroy(g));
+                g = nullptr;
+            }
+        c.gsteps = nsteps;
+        c.gshape = c.shape;
+    }
+    hipGraphExec_t &g = c.graph[c.cur];
+    if (!g) {
+        const int cur0 = c.cur;
+        const int64_t itc0 = c.itc;
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        const int32_t rc = launch_steps(c, nsteps);
+        const hipError_t e = hipStreamEndCapture(c.stream, &graph);
+        c.cur = cur0;
+        c.itc = itc0;
+        if (rc || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc ? rc : hip_fail(e, "step graph capture");
+        }
+        const hipError_t ei = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            g = nullptr;
+            return hip_fail(ei, "step graph instantiate");
+        }
+    }
+    HIP_TRY(hipGraphLaunch(g, c.stream));
+    c.cur ^= (int)(nsteps & 1);
+    c.itc += nsteps;
     return LBM_OK;
 }
 

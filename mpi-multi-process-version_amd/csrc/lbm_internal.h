@@ -55,6 +55,12 @@ struct Ctx {
     LaunchShape shape{};
     StepArgs<float> a32{};
     StepArgs<double> a64{};
+    // Small grids, where a launch costs more than the step: one check
+    // interval of steps (gsteps) replayed as a HIP graph, graph[p] starting
+    // from buffer p, captured for launch shape gshape.
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int64_t gsteps = 0;
+    LaunchShape gshape{};
 };
 
 // Fills the per-launch constants of both argument blocks (pointers to the

@@ -381,3 +381,38 @@ def test_lds_zmarch_variant_is_rejected_where_it_cannot_run(hip):
         with pytest.raises(hip.LbmError):
             s.set_launch(bx, by, variant=1)
         s.close()
+
+
+@pytest.mark.parametrize("dim,n,prec,launch", [
+    (2, 33, "fp32", None), (2, 64, "fp64", None), (3, 16, "fp32", (64, 4, 0, 1)),
+    (2, 65, "fp32", (64, 1, 1, 1))])
+def test_graph_replays_match_direct_launches(hip, dim, n, prec, launch):
+    """Grids up to 2^20 nodes replay a check interval (check_every = 2000
+    steps) as a HIP graph of the same launches; any other step count is
+    launched directly.  Three replays in a row, with direct launches, checks
+    and field reads between them, must each match the direct run bit for bit."""
+    print(f"\n[graph] dim={dim} n={n} {prec} launch={launch}", flush=True)
+    p = hip.PREC_FP64 if prec == "fp64" else hip.PREC_FP32_SHIFTED
+    a = hip.Solver(dim, n, 1e5, precision=p)
+    b = hip.Solver(dim, n, 1e5, precision=p)
+    for s in (a, b):
+        if launch:
+            s.set_launch(launch[0], launch[1], remap=launch[2], variant=launch[3])
+        s.init_rest()
+    for r in range(3):
+        a.step(2000)                  # graph replay
+        b.step(1200)                  # direct launches
+        b.step(800)
+        assert a.itc == b.itc == 4000 * r + 2000
+        fa, fb = a.fields(), b.fields()
+        for name, x, y in zip(("rho", "u", "v", "w", "T"), fa, fb):
+            assert np.array_equal(x, y), (r, name)
+        assert a.check() == b.check()
+        a.step(1)                     # direct work between replays
+        b.step(1)
+        a.step(1999)
+        b.step(1999)
+    for x, y in zip(a.fields(), b.fields()):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
