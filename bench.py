@@ -28,6 +28,8 @@ WORKLOADS = {
     "2d1025": dict(dim=2, n=1025, Ra=1e6, label="cavity2d_1025_ra1e6_d2q9_d2q5"),
     "batch64": dict(dim=2, count=64, seed=20261203,
                     label="batch_64_cavities2d_n65-513_A1-2.5_Ra1e3-1e6_Pr0.7-7"),
+    "steady129": dict(dim=2, n=129, Ra=1e4, steady=True,
+                      label="cavity2d_129_ra1e4_to_steady_state"),
     "warm64": dict(dim=2, n=64, held_out=100, seed=7,
                    label="warm_start_64x64_cavities_Ra1e3-1e6_Pr0.7-7_A1-2.5"),
 }
@@ -148,6 +150,75 @@ def run_batch(args, cfg):
     print(json.dumps(out), flush=True)
 
 
+def run_steady(args, cfg):
+    """The reference's own case run to steady state: 2D 129^2, Ra = 1e4,
+    Pr = 0.71, from the reference IC to both residuals <= 1e-8 in FP64,
+    checking every 2000 steps, then the FP32 shifted path for the same number
+    of steps with its Nu against the FP64 one.  The value is the FP64 wall
+    time from lbm_create to the converged check."""
+    import lbmhip
+    runs = {}
+    itc_max = 2000000
+    for name, prec, eps in (("fp64", lbmhip.PREC_FP64, 1e-8),
+                            ("fp32_same_steps", lbmhip.PREC_FP32_SHIFTED, 1e-8)):
+        t0 = time.perf_counter()
+        s = lbmhip.Solver(2, cfg["n"], cfg["Ra"], precision=prec, epsU=eps, epsT=eps,
+                          itc_max=itc_max, device=0)
+        s.init_rest()
+        status, itc, eu, et = s.run()
+        wall = time.perf_counter() - t0
+        nu = s.nusselt()
+        s.close()
+        runs[name] = {"eps": eps, "status": status, "itc": itc, "errU": eu, "errT": et,
+                      "wall_s": round(wall, 4), "nu_hot": nu[0], "nu_cold": nu[1]}
+        # FP32 shifted populations level off near 1e-6 in the velocity residual,
+        # so the FP32 leg runs the FP64 run's step count and is judged by its Nu
+        itc_max = itc
+    r = runs["fp64"]
+    f = runs["fp32_same_steps"]
+    f["nu_rel_err_vs_fp64"] = max(abs(f["nu_hot"] - r["nu_hot"]) / abs(r["nu_hot"]),
+                                  abs(f["nu_cold"] - r["nu_cold"]) / abs(r["nu_cold"]))
+    nodes = cfg["n"] ** 2
+    out = {
+        "metric": "time to steady state (wall, lbm_create to the converged check), 1 GPU",
+        "value": r["wall_s"], "unit": "s", "n_gpus": 1, "steps": r["itc"], "warmup": 0,
+        "ms_per_step": round(1e3 * r["wall_s"] / r["itc"], 5), "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: reference initial condition (rest, linear T)",
+        "config": {"workload": cfg["label"], "nodes": nodes, "grid": "erf A=3.2",
+                   "Ra": cfg["Ra"], "Pr": 0.71, "check_every": 2000,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(224 * nodes * r["itc"] / r["wall_s"] / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(224 * nodes * r["itc"] / r["wall_s"] / 1e9 / HBM_PEAK_GBS, 5),
+                     "traffic": None,
+                     "note": "16 641 nodes: each step is one short kernel (its latency, not HBM, "
+                             "bounds it); check intervals replay as HIP graphs"},
+        "check": runs,
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import lbm_oracle
+        lbm_oracle.set_threads(args.cpu_threads)
+        steps = args.cpu_steps or 20000
+        o = lbm_oracle.Oracle(2, cfg["n"], cfg["Ra"])
+        o.init_rest()
+        o.step(1)
+        t = time.perf_counter()
+        o.step(steps)
+        dt = time.perf_counter() - t
+        o.close()
+        out["cpu_baseline"] = {
+            "value": round(dt / steps * r["itc"], 2), "unit": "s", "cores": args.cpu_threads,
+            "kind": "port",
+            "sample": f"{steps} steps of the same cavity (FP64 CPU oracle, {args.cpu_threads} "
+                      f"OpenMP threads, {dt:.1f} s), scaled to the {r['itc']} steps to steady "
+                      f"state"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def run_warm(args, cfg):
     """The learned warm start against the cold start on held-out cavities:
     each cavity runs from rest and from the trained network's prediction to
@@ -261,6 +332,9 @@ def main():
     cfg = WORKLOADS[args.workload]
     if "held_out" in cfg:
         run_warm(args, cfg)
+        return
+    if cfg.get("steady"):
+        run_steady(args, cfg)
         return
     if "count" in cfg:
         if args.precision != "fp32" or args.launch:
