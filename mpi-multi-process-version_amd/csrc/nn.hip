@@ -7,7 +7,8 @@
 //             dcol[K][N]  = W[cout][K]^T . dZ[cout][N], then col2im
 // with N = batch x output pixels.  The product kernel multiplies BF16
 // operands with v_mfma_f32_16x16x16_bf16 and accumulates in FP32; one wave
-// computes one 16 x 16 tile of the result.  Activations are FP32 in
+// computes a 64 x 16 block of the result (four 16 x 16 tiles sharing the
+// right operand's fragment).  Activations are FP32 in
 // channel-major layout [C][B][H][W], so a channel concatenation is a
 // contiguous stack and a layer's product output is its activation.  All
 // reductions run in a fixed order, so every call is deterministic.
@@ -51,32 +52,52 @@ struct GemmArgs {
 };
 
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+    // one wave per 64 x 16 block of C: up to four 16 x 16 MFMA tiles down
+    // the m axis share each B fragment, so B (the large operand) is read
+    // once per 64 rows of C instead of once per 16
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int tn = (g.N + 15) >> 4, tm = (g.M + 15) >> 4;
-    if (wave >= tn * tm) return;   // whole waves only
-    const int m0 = (wave / tn) << 4, n0 = (wave % tn) << 4;
+    const int tn = (g.N + 15) >> 4, tg = (g.M + 63) >> 6;
+    if (wave >= tn * tg) return;   // whole waves only
+    const int mb = (wave / tn) << 6, n0 = (wave % tn) << 4;
     const int kb = blockIdx.z * g.kchunk, ke = min(g.K, kb + g.kchunk);
     const int r = lane & 15, q = lane >> 4;
-    const long m = m0 + r, n = n0 + r;
-    const bool mok = m < g.M, nok = n < g.N;
-    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const long n = n0 + r;
+    const bool nok = n < g.N;
+    const int nt = min(4, (g.M - mb + 15) >> 4);   // m tiles in this block (wave-uniform)
+    v4f acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     for (int k0 = kb; k0 < ke; k0 += 16) {
-        v4s a, b;
+        v4s b;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const long k = k0 + 4 * q + e;
-            const bool kin = k < ke;
-            a[e] = (kin && mok) ? (short)g.A[m * g.as_m + k * g.as_k] : (short)0;
-            b[e] = (kin && nok) ? (short)g.B[k * g.bs_k + n * g.bs_n] : (short)0;
+            b[e] = (k < ke && nok) ? (short)g.B[k * g.bs_k + n * g.bs_n] : (short)0;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if (t >= nt) break;
+            const long m = mb + 16 * t + r;
+            const bool mok = m < g.M;
+            v4s a;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const long k = k0 + 4 * q + e;
+                a[e] = (k < ke && mok) ? (short)g.A[m * g.as_m + k * g.as_k] : (short)0;
+            }
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
     }
     float *C = g.C + (long)blockIdx.z * g.M * g.ldc;
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-        const long mi = m0 + 4 * q + e, nj = n0 + r;
-        if (mi < g.M && nj < g.N) C[mi * g.ldc + nj] = acc[e];
+    for (int t = 0; t < 4; t++) {
+        if (t >= nt) break;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const long mi = mb + 16 * t + 4 * q + e, nj = n0 + r;
+            if (mi < g.M && nj < g.N) C[mi * g.ldc + nj] = acc[t][e];
+        }
     }
 }
 
@@ -162,19 +183,31 @@ __global__ void k_to_bf16(const float *x, bf16 *y, long n) {
         y[t] = to_bf16(x[t]);
 }
 
-// out[r] = sum_n X[r N + n]: one block per row, fixed order
-__global__ __launch_bounds__(256) void k_rowsum(const float *X, float *out, long N) {
+// part[r P + p] = sum of row r of X over the p-th of P equal column ranges:
+// blocks (r, p), fixed order within each
+__global__ __launch_bounds__(256) void k_rowsum_part(const float *X, float *part, long N) {
     __shared__ float sh[256];
+    const long len = (N + gridDim.y - 1) / gridDim.y;
+    const long c0 = (long)blockIdx.y * len, c1 = min(N, c0 + len);
     const float *row = X + (long)blockIdx.x * N;
     float s = 0.0f;
-    for (long n = threadIdx.x; n < N; n += 256) s += row[n];
+    for (long c = c0 + threadIdx.x; c < c1; c += 256) s += row[c];
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) part[(long)blockIdx.x * gridDim.y + blockIdx.y] = sh[0];
+}
+
+// out[r] = sum_p part[r P + p], p in order
+__global__ void k_rowsum_final(const float *part, float *out, int rows, int P) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    float s = 0.0f;
+    for (int p = 0; p < P; p++) s += part[(long)r * P + p];
+    out[r] = s;
 }
 
 // nearest x2 upsampling of [C][B][h][w] and its adjoint
@@ -330,13 +363,14 @@ hipError_t gemm(lbm_nn n, int M, int N, int K, const bf16 *A, long as_m, long as
                 const bf16 *B, long bs_k, long bs_n, float *C, long ldc) {
     // split the k range when the output is small and k is long
     const long tiles = (long)((M + 15) / 16) * ((N + 15) / 16);
+    const long blocks = (long)((M + 63) / 64) * ((N + 15) / 16);   // one wave each
     int splits = 1;
     if (tiles < 1024 && K >= 4096) splits = (int)std::min<long>(64, std::max<long>(1, K / 2048));
     const int kchunk = ((K + splits - 1) / splits + 15) / 16 * 16;
     splits = (K + kchunk - 1) / kchunk;
     nn::GemmArgs g{A, as_m, as_k, B, bs_k, bs_n, splits > 1 ? n->part : C,
                    splits > 1 ? (long)N : ldc, M, N, K, kchunk};
-    dim3 grid((unsigned)((tiles + 3) / 4), 1, (unsigned)splits);
+    dim3 grid((unsigned)((blocks + 3) / 4), 1, (unsigned)splits);
     hipLaunchKernelGGL(nn::k_gemm, grid, dim3(256), 0, n->s, g);
     if (splits > 1) {
         const long mn = (long)M * N;
@@ -369,8 +403,12 @@ hipError_t conv_bwd(lbm_nn n, const ConvLayer &l, int B, const float *X, int H, 
     if (l.relu)
         hipLaunchKernelGGL(nn::k_relu_bwd, dim3(nn::grid_for(l.cout * N)), dim3(256), 0, n->s, dA,
                            A, (long)l.cout * N);
-    hipLaunchKernelGGL(nn::k_rowsum, dim3(l.cout), dim3(256), 0, n->s, (const float *)dA,
-                       n->g + l.boff, N);
+    // bias gradient: row sums of dZ, split into P column ranges per row
+    const int P = (int)std::min<long>(64, std::max<long>(1, (N + 8191) / 8192));
+    hipLaunchKernelGGL(nn::k_rowsum_part, dim3(l.cout, P), dim3(256), 0, n->s,
+                       (const float *)dA, n->part, N);
+    hipLaunchKernelGGL(nn::k_rowsum_final, dim3((l.cout + 63) / 64), dim3(64), 0, n->s,
+                       (const float *)n->part, n->g + l.boff, l.cout, P);
     hipLaunchKernelGGL(nn::k_to_bf16, dim3(nn::grid_for(l.cout * N)), dim3(256), 0, n->s,
                        (const float *)dA, n->dzb, (long)l.cout * N);
     hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(K * N)), dim3(256), 0, n->s, X, n->col,

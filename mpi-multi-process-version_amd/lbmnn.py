@@ -145,3 +145,62 @@ def fields(y, V):
     """(u, v, T) in LB units from one network output y[3][G][G]."""
     y = np.asarray(y, dtype=np.float64)
     return y[0] * V, y[1] * V, y[2]
+
+
+def erf_nodes(n, A):
+    """Node coordinates x[n] and cavity size H (LB units) of the erf grid of
+    n nodes with stretch A, the map of lbm_create (after
+    natural-convection_2dim_mono.f90:85-116): X_i = (erf(A (i/(n+1) - 1/2))
+    / erf(A/2) + 1) / 2 for i = 0..n+1, dx0 = X_1 - X_0, L = 1/dx0,
+    x_i = (X_i - dx0/2) L for i = 1..n, H = L - 1."""
+    from math import erf
+    X = np.array([0.5 * (erf(A * (i / (n + 1.0) - 0.5)) / erf(0.5 * A) + 1.0)
+                  for i in range(n + 2)])
+    dx0 = X[1] - X[0]
+    L = 1.0 / dx0
+    return (X[1:n + 1] - 0.5 * dx0) * L, L - 1.0
+
+
+def _lerp_matrix(src, dst):
+    """Dense [len(dst)][len(src)] matrix of piecewise-linear interpolation
+    from increasing points src to points dst inside [src[0], src[-1]]."""
+    src, dst = np.asarray(src, np.float64), np.asarray(dst, np.float64)
+    idx = np.clip(np.searchsorted(src, dst, side="right") - 1, 0, len(src) - 2)
+    w = (dst - src[idx]) / (src[idx + 1] - src[idx])
+    M = np.zeros((len(dst), len(src)))
+    rows = np.arange(len(dst))
+    M[rows, idx] = 1.0 - w
+    M[rows, idx + 1] = w
+    return M
+
+
+def resample(y, s_src, s_dst, t_hot=1.0, t_cold=0.0):
+    """A network output y[3][G][G] (u/V, v/V, T on the nodes s_src = x/H of
+    its own grid, both axes alike) on another grid of the same cavity, the
+    nodes s_dst = x/H: piecewise-linear in each axis over the nodes plus the
+    four walls, where the wall values are the boundary conditions of the
+    cavity (u = v = 0 on every wall, T = t_hot at x = 0 and t_cold at
+    x = H, zero normal gradient of T at y = 0 and y = H).  Returns
+    [3][len(s_dst)][len(s_dst)]."""
+    y = np.asarray(y, np.float64)
+    G = y.shape[-1]
+    s = np.concatenate(([0.0], np.asarray(s_src, np.float64), [1.0]))
+    aug = np.zeros((3, G + 2, G + 2))
+    aug[:, 1:-1, 1:-1] = y
+    aug[2, 1:-1, 0] = t_hot
+    aug[2, 1:-1, -1] = t_cold
+    aug[2, 0, :] = aug[2, 1, :]
+    aug[2, -1, :] = aug[2, -2, :]
+    M = _lerp_matrix(s, s_dst)
+    return np.stack([M @ aug[c] @ M.T for c in range(3)])
+
+
+def extrude(u, v, T, n):
+    """The 3D cube's fields from a 2D prediction on its (y, z) plane: the
+    3D cavity has its heated walls at y = 0 and y = H and gravity along +z,
+    so the 2D (x, y) become (y, z).  Every x slice gets the same plane:
+    u3 = 0, v3 = u, w3 = v, T3 = T, each [n][n][n] with i fastest."""
+    shape = (n, n, n)
+    return (np.zeros(shape), np.broadcast_to(u[:, :, None], shape).copy(),
+            np.broadcast_to(v[:, :, None], shape).copy(),
+            np.broadcast_to(T[:, :, None], shape).copy())

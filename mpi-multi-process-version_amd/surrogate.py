@@ -15,9 +15,17 @@ Every step runs on the GPU: the batched solver generates the steady fields
 (one launch per time step for all cavities), the network trains with its
 own forward, backward and Adam kernels, and inference is one forward call.
 
+Grids other than the network's G x G (2D cavities of other sizes, the 3D
+cube) get the network's output on G erf nodes resampled onto their nodes,
+and a cube the resulting plane extruded along its adiabatic x axis
+(warm_fields).  Training data from larger grids is sampled at the network's
+nodes the same way (generate).
+
 Command line (prints one JSON record per stage):
+    python surrogate.py generate --out data.npz --n 257 [--count 256 ...]
     python surrogate.py train --out weights.npz [--count 2048 ...]
-    python surrogate.py compare --weights weights.npz [--count 100 ...]
+    python surrogate.py train --out w.npz --data data.npz [--init warm64.npz]
+    python surrogate.py compare --weights weights.npz [--n 257 --count 100 ...]
 """
 import argparse
 import json
@@ -59,10 +67,23 @@ def _batch(cavities, eps, itc_max):
                         check_every=CHECK_EVERY)
 
 
+def net_inputs(cavity, s, G=GRID):
+    """The network input [4][G][G] for cavity (Ra, Pr, stretch) on Solver s,
+    and the nodes x/H it is given on: None when those are the solver's own
+    nodes (a 2D grid of G nodes), else the G nodes of the erf grid with the
+    cavity's stretch clipped to the trained range, between which the
+    network's output is resampled onto the solver's grid (lbmnn.resample)."""
+    if s.dim == 2 and s.n == G:
+        return lbmnn.inputs(cavity["Ra"], cavity["Pr"], s.grid(), s.info()["H"]), None
+    xc, Hc = lbmnn.erf_nodes(G, min(max(cavity["stretch"], STRETCH[0]), STRETCH[1]))
+    return lbmnn.inputs(cavity["Ra"], cavity["Pr"], xc, Hc), xc / Hc
+
+
 def network_inputs(batch, cavities):
-    """x[b][4][G][G] of the cavities of a Batch (their own grids)."""
-    return np.stack([lbmnn.inputs(c["Ra"], c["Pr"], batch[i].grid(), batch[i].info()["H"])
-                     for i, c in enumerate(cavities)])
+    """x[b][4][G][G] of the cavities of a Batch and their node lists (see
+    net_inputs)."""
+    got = [net_inputs(c, batch[i]) for i, c in enumerate(cavities)]
+    return np.stack([g[0] for g in got]), [g[1] for g in got]
 
 
 def solve(cavities, eps=EPS, itc_max=600000, net=None, keep_fields=False):
@@ -72,13 +93,15 @@ def solve(cavities, eps=EPS, itc_max=600000, net=None, keep_fields=False):
     `predict` (inputs, forward, lbm_init_fields), `run` (the time loop)."""
     b = _batch(cavities, eps, itc_max)
     t0 = time.perf_counter()
-    x = network_inputs(b, cavities)
+    x, nodes = network_inputs(b, cavities)
     if net is None:
         b.init_rest()
     else:
         y = predict(net, x)
         for i in range(len(cavities)):
-            u, v, T = lbmnn.fields(y[i], lbmnn.velocity_scale(b[i].info()))
+            yi = y[i] if nodes[i] is None else lbmnn.resample(
+                y[i], nodes[i], b[i].grid() / b[i].info()["H"])
+            u, v, T = lbmnn.fields(yi, lbmnn.velocity_scale(b[i].info()))
             b[i].init_fields(u, v, None, T)
     b[0].synchronize()
     t1 = time.perf_counter()
@@ -90,7 +113,10 @@ def solve(cavities, eps=EPS, itc_max=600000, net=None, keep_fields=False):
         t = np.zeros((len(cavities), 3, GRID, GRID), dtype=np.float32)
         for i in range(len(cavities)):
             _, u, v, _, T = b[i].fields()
-            t[i] = lbmnn.targets(u, v, T, lbmnn.velocity_scale(b[i].info()))
+            ti = lbmnn.targets(u, v, T, lbmnn.velocity_scale(b[i].info()))
+            # fields of a larger grid are sampled at the network's nodes
+            t[i] = ti if nodes[i] is None else lbmnn.resample(
+                ti, b[i].grid() / b[i].info()["H"], nodes[i])
         out["y"] = t
     b.close()
     return out
@@ -102,12 +128,15 @@ def predict(net, x):
                            for i in range(0, len(x), net.batch_max)])
 
 
-def generate(count, seed, eps=EPS, itc_max=600000, per_batch=256, keep_tol=1e-4, log=print):
-    """Steady fields of `count` seeded cavities.  A cavity is kept if it
-    reached the stopping rule, or ran to itc_max with errU and errT below
-    keep_tol (a slowly settling cavity, already steady to that tolerance);
-    diverged cavities are dropped.  Returns (x, y, record)."""
-    cavs = sample_cavities(count, seed)
+def generate(count, seed, eps=EPS, itc_max=600000, per_batch=256, keep_tol=1e-4, n=GRID,
+             log=print):
+    """Steady fields of `count` seeded cavities on n x n grids, as network
+    inputs and targets on the network's G x G nodes (sampled there from a
+    larger grid, see net_inputs).  A cavity is kept if it reached the
+    stopping rule, or ran to itc_max with errU and errT below keep_tol (a
+    slowly settling cavity, already steady to that tolerance); diverged
+    cavities are dropped.  Returns (x, y, record)."""
+    cavs = sample_cavities(count, seed, n=n)
     xs, ys, keep, itcs = [], [], [], []
     t0 = time.perf_counter()
     for s in range(0, count, per_batch):
@@ -170,6 +199,23 @@ def load_net(path=WEIGHTS, batch_max=32, device=0):
     return net
 
 
+def warm_fields(net, cavity, s):
+    """(u, v, w, T) for lbm_init_fields() of the Solver s from the network's
+    prediction for cavity (Ra, Pr, stretch), on the nodes net_inputs()
+    names and resampled onto the solver's grid where those differ; a 3D cube
+    gets that plane extruded along x (lbmnn.extrude), the 2D cavity being
+    the cube's (y, z) cross-section."""
+    info = s.info()
+    x, nodes = net_inputs(cavity, s, net.grid)
+    y = net.forward(x[None])[0]
+    if nodes is not None:
+        y = lbmnn.resample(y, nodes, s.grid() / info["H"])
+    u, v, T = lbmnn.fields(y, lbmnn.velocity_scale(info))
+    if s.dim == 2:
+        return u, v, None, T
+    return lbmnn.extrude(u, v, T, s.n)
+
+
 def traced_run(cavity, net=None, eps=EPS, itc_max=600000):
     """One cavity to the stopping rule (the loop of lbm_run(): check every
     CHECK_EVERY steps, stop when errU and errT are both <= eps), from rest
@@ -177,15 +223,16 @@ def traced_run(cavity, net=None, eps=EPS, itc_max=600000):
     count, the wall time since the start (inference and lbm_init_fields
     included) and the hot-wall Nusselt number."""
     c = dict(cavity)
-    s = lbmhip.Solver(2, c.pop("n"), c.pop("Ra"), Pr=c.pop("Pr"), stretch=c.pop("stretch"),
-                      epsU=eps, epsT=eps, check_every=CHECK_EVERY, itc_max=itc_max)
+    s = lbmhip.Solver(c.pop("dim", 2), c.pop("n"), c.pop("Ra"), Pr=c.pop("Pr"),
+                      stretch=c.pop("stretch"), epsU=eps, epsT=eps, check_every=CHECK_EVERY,
+                      itc_max=itc_max)
+    if s.dim == 3:
+        s.set_launch(64, 4, remap=2, variant=1)   # the LDS z-march
     t0 = time.perf_counter()
     if net is None:
         s.init_rest()
     else:
-        x = lbmnn.inputs(cavity["Ra"], cavity["Pr"], s.grid(), s.info()["H"])
-        u, v, T = lbmnn.fields(net.forward(x[None])[0], lbmnn.velocity_scale(s.info()))
-        s.init_fields(u, v, None, T)
+        s.init_fields(*warm_fields(net, cavity, s))
     status, trace = lbmhip.LBM_ERR_NOT_CONVERGED, []
     while s.itc < itc_max:
         s.step(min(CHECK_EVERY, itc_max - s.itc))
@@ -276,11 +323,39 @@ def summarise(rows):
         cold_band_s=tot("cold_band_s", banded), warm_band_s=tot("warm_band_s", banded))
 
 
+def save_data(path, x, y, record):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, x=x, y=y, record=np.array(json.dumps(record)))
+
+
+def load_data(paths):
+    """(x, y, records) of the files save_data() wrote, concatenated."""
+    xs, ys, recs = [], [], []
+    for p in paths:
+        with np.load(p, allow_pickle=False) as z:
+            xs.append(z["x"])
+            ys.append(z["y"])
+            recs.append(json.loads(str(z["record"])))
+    return np.concatenate(xs), np.concatenate(ys), recs
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
+    g = sub.add_parser("generate")
+    g.add_argument("--out", required=True)
+    g.add_argument("--n", type=int, default=GRID)
+    g.add_argument("--count", type=int, default=256)
+    g.add_argument("--seed", type=int, default=20261203)
+    g.add_argument("--itc-max", type=int, default=600000)
+    g.add_argument("--per-batch", type=int, default=256)
+    g.add_argument("--keep-tol", type=float, default=1e-4)
     t = sub.add_parser("train")
     t.add_argument("--out", default=WEIGHTS)
+    t.add_argument("--data", nargs="*", default=None,
+                   help="train on these generate files instead of generating")
+    t.add_argument("--init", default=None, help="start from these weights")
+    t.add_argument("--n", type=int, default=GRID)
     t.add_argument("--count", type=int, default=2048)
     t.add_argument("--seed", type=int, default=20261203)
     t.add_argument("--epochs", type=int, default=200)
@@ -291,16 +366,32 @@ def main(argv=None):
     t.add_argument("--keep-tol", type=float, default=1e-4)
     c = sub.add_parser("compare")
     c.add_argument("--weights", default=WEIGHTS)
+    c.add_argument("--n", type=int, default=GRID)
     c.add_argument("--count", type=int, default=100)
     c.add_argument("--seed", type=int, default=7)
     c.add_argument("--itc-max", type=int, default=600000)
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
-    if a.cmd == "train":
+    if a.cmd == "generate":
         x, y, rec = generate(a.count, a.seed, itc_max=a.itc_max, per_batch=a.per_batch,
-                             keep_tol=a.keep_tol, log=log)
+                             keep_tol=a.keep_tol, n=a.n, log=log)
+        rec.update(n=a.n)
+        save_data(a.out, x, y, rec)
+        log(json.dumps(dict(stage="saved", path=a.out, **rec)))
+    elif a.cmd == "train":
+        if a.data:
+            x, y, recs = load_data(a.data)
+            rec = dict(data=[dict(path=os.path.basename(p), **r) for p, r in zip(a.data, recs)])
+        else:
+            x, y, rec = generate(a.count, a.seed, itc_max=a.itc_max, per_batch=a.per_batch,
+                                 keep_tol=a.keep_tol, n=a.n, log=log)
+            rec.update(n=a.n)
         nval = max(1, len(x) // 16)
-        net = lbmnn.Net(grid=GRID, batch_max=a.batch, seed=a.seed)
+        if a.init:
+            net = load_net(a.init, batch_max=a.batch)
+            rec.update(init=os.path.basename(a.init))
+        else:
+            net = lbmnn.Net(grid=GRID, batch_max=a.batch, seed=a.seed)
         t0 = time.perf_counter()
         hist = train(net, x[nval:], y[nval:], a.epochs, a.batch, a.lr,
                      val=(x[:nval], y[:nval]), log=log)
@@ -310,7 +401,8 @@ def main(argv=None):
         log(json.dumps(dict(stage="saved", path=a.out, **rec)))
     else:
         net = load_net(a.weights)
-        _, summary = compare(net, sample_cavities(a.count, a.seed), itc_max=a.itc_max, log=log)
+        _, summary = compare(net, sample_cavities(a.count, a.seed, n=a.n), itc_max=a.itc_max,
+                             log=log)
         log(json.dumps(dict(stage="summary", seed=a.seed, **summary)))
 
 

@@ -78,3 +78,44 @@ def test_untrained_network_gains_less(trained):
           flush=True)
     assert s1["fallbacks"] == 0
     assert s0["band_wall_ratio"] is None or s1["band_wall_ratio"] > 2 * s0["band_wall_ratio"]
+
+
+@pytest.fixture(scope="module")
+def trained257():
+    path = os.path.join(os.path.dirname(S.WEIGHTS), "warm257.npz")
+    assert os.path.exists(path), path
+    net = S.load_net(path)
+    yield net
+    net.close()
+
+
+def test_warm_start_resampled_onto_257_grids(trained257):
+    """The network runs on its 64 x 64 erf grid and its output is resampled
+    onto the 257 x 257 nodes (weights fine-tuned on 257 x 257 steady fields
+    sampled at the network's nodes)."""
+    cavs = S.sample_cavities(3, 7, n=257)
+    print("\n[surrogate 257] 3 held-out cavities, seed 7", flush=True)
+    rows, s = S.compare(trained257, cavs, itc_max=2000000)
+    for r in rows:
+        print(f"  Ra {r['Ra']:.3g} Pr {r['Pr']:.2f}: Nu band at {r['cold_band_itc']} -> "
+              f"{r['warm_band_itc']}, dNu {r['nu_rel_dev']:.1e}", flush=True)
+    print(f"  steps to the Nu band {s['band_itc_ratio']:.2f}x, wall {s['band_wall_ratio']:.2f}x",
+          flush=True)
+    assert s["judged"] == len(cavs) and s["fallbacks"] == 0
+    assert s["nu_rel_dev_max"] <= 1e-3
+    assert s["band_cavities"] == len(cavs)
+    assert s["band_itc_ratio"] >= 1.5
+
+
+def test_warm_start_extruded_onto_a_cube(trained257):
+    """The 3D cube from the 2D prediction of its (y, z) cross-section: the
+    solver reaches the cold start's steady Nusselt number from it."""
+    cav = dict(dim=3, n=32, Ra=1e5, Pr=0.71, stretch=3.2)
+    print("\n[surrogate cube] 32^3, Ra 1e5", flush=True)
+    rows, s = S.compare(trained257, [cav], itc_max=1000000)
+    r = rows[0]
+    print(f"  itc {r['cold_itc']} -> {r['warm_itc']}, Nu band at {r['cold_band_itc']} -> "
+          f"{r['warm_band_itc']}, dNu {r['nu_rel_dev']:.1e}", flush=True)
+    assert s["judged"] == 1 and s["fallbacks"] == 0
+    assert s["nu_rel_dev_max"] <= 1e-3
+    assert r["warm_band_itc"] <= r["cold_band_itc"]

@@ -47,6 +47,68 @@ def test_targets_scale_velocity_by_a_quarter_of_free_fall():
         assert np.allclose(a, b, rtol=1e-6)
 
 
+def test_erf_nodes_match_the_oracle_grid(oracle_mod):
+    for n, A in ((64, 1.0), (64, 2.5), (129, 3.2), (257, 1.7), (128, 3.2)):
+        o = oracle_mod.Oracle(2, n, 1e4, A=A)
+        x, H = lbmnn.erf_nodes(n, A)
+        assert np.allclose(x, o.coords(), rtol=1e-13, atol=0)
+        assert np.isclose(H, o.params()["H"], rtol=1e-13)
+        o.close()
+
+
+def test_resample_is_exact_for_fields_linear_in_each_axis():
+    """Piecewise-linear interpolation reproduces a linear function of the
+    coordinate exactly, over the nodes and the walls; so T = 1 - x/H (which
+    meets the hot, cold and adiabatic wall values resample() adds) comes
+    through unchanged on any target grid."""
+    G = 16
+    xs, Hs = lbmnn.erf_nodes(G, 2.0)
+    xt, Ht = lbmnn.erf_nodes(41, 1.3)
+    ss, st = xs / Hs, xt / Ht
+    M = lbmnn._lerp_matrix(np.concatenate(([0.0], ss, [1.0])), st)
+    assert np.allclose(M.sum(axis=1), 1.0) and (M >= 0).all()
+    assert (np.count_nonzero(M, axis=1) <= 2).all()
+    f = lambda x: 0.3 - 1.7 * x   # noqa: E731
+    src = np.concatenate(([0.0], ss, [1.0]))
+    assert np.allclose(M @ f(src), f(st), rtol=0, atol=1e-14)
+    # T linear in x from 1 (hot wall) to 0 (cold wall), constant in y: exact
+    y = np.zeros((3, G, G))
+    y[2] = 1.0 - ss[None, :]
+    r = lbmnn.resample(y, ss, st)
+    assert r.shape == (3, 41, 41)
+    assert np.allclose(r[2], 1.0 - st[None, :], atol=1e-14)
+    assert np.all(r[:2] == 0.0)
+
+
+def test_resample_keeps_the_source_nodes_and_the_wall_values():
+    G = 16
+    xs, Hs = lbmnn.erf_nodes(G, 2.0)
+    ss = xs / Hs
+    rng = np.random.default_rng(3)
+    y = rng.standard_normal((3, G, G))
+    assert np.allclose(lbmnn.resample(y, ss, ss), y, rtol=0, atol=1e-15)
+    # a target node half-way between the wall and the first source node gets
+    # the mean of the wall value and that node's value
+    st = np.array([0.5 * ss[0], ss[3]])
+    r = lbmnn.resample(y, ss, st)
+    assert np.isclose(r[0, 1, 0], 0.5 * y[0, 3, 0])          # u: 0 on the wall x = 0
+    assert np.isclose(r[2, 1, 0], 0.5 * (1.0 + y[2, 3, 0]))  # T: 1 on the hot wall
+    assert np.isclose(r[2, 0, 1], y[2, 0, 3])                # T: adiabatic wall y = 0
+    assert np.isclose(r[1, 0, 1], 0.5 * y[1, 0, 3])          # v: 0 on the wall y = 0
+
+
+def test_extrude_puts_the_plane_on_the_cubes_y_z_axes():
+    n = 5
+    u = np.arange(25.0).reshape(5, 5)
+    v, T = -u, u / 25.0
+    u3, v3, w3, T3 = lbmnn.extrude(u, v, T, n)
+    assert u3.shape == v3.shape == w3.shape == T3.shape == (n, n, n)
+    assert np.all(u3 == 0)
+    k, j, i = 3, 1, 4   # [k][j][i], i fastest: z index k, heated-axis index j
+    assert v3[k, j, i] == u[k, j] and w3[k, j, i] == v[k, j] and T3[k, j, i] == T[k, j]
+    assert np.all(T3 == T3[:, :, :1])
+
+
 def test_band_entry_is_the_last_entry_into_the_band():
     nu = np.array([1.0, 1.5, 2.0, 2.0025, 1.99, 2.001, 2.0004, 2.0])
     trace = np.stack([np.arange(1, 9) * 2000, np.arange(1, 9) * 0.5, nu], axis=1)
