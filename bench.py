@@ -35,7 +35,7 @@ WORKLOADS = {
     "warm257": dict(dim=2, n=257, held_out=24, seed=7, weights="warm257.npz", itc_max=2000000,
                     label="warm_start_257x257_cavities_Ra1e3-1e6_Pr0.7-7_A1-2.5"),
     "warm3d128": dict(dim=3, n=128, held_out=1, cube=dict(Ra=1e5, Pr=0.71, stretch=3.2),
-                      weights="warm257.npz", itc_max=1500000,
+                      weights="warm257.npz", itc_max=1200000, eps=2e-6,
                       label="warm_start_128^3_cube_Ra1e5_Pr0.71_A3.2_slice_extruded"),
 }
 
@@ -232,7 +232,9 @@ def run_warm(args, cfg):
     converged value (network inference and initialisation included).
     Cavities larger than the network's 64 x 64 grid get its output
     resampled onto their nodes; the 3D cube gets the 2D prediction of its
-    (y, z) cross-section extruded along x."""
+    (y, z) cross-section extruded along x.  The cube's rule is eps = 2e-6:
+    at 128^3 the FP32 velocity residual wanders between 1e-6 and 2e-6 for
+    hundreds of thousands of steps before it settles below 1e-6."""
     import numpy as np
     import lbmhip
     import surrogate
@@ -246,7 +248,8 @@ def run_warm(args, cfg):
     else:
         cavs = surrogate.sample_cavities(cfg["held_out"], cfg["seed"], n=n)
     t0 = time.perf_counter()
-    rows, s = surrogate.compare(net, cavs, itc_max=cfg["itc_max"],
+    eps = cfg.get("eps", surrogate.EPS)
+    rows, s = surrogate.compare(net, cavs, eps=eps, itc_max=cfg["itc_max"],
                                 log=lambda r: print(r, file=sys.stderr, flush=True))
     wall = time.perf_counter() - t0
     net.close()
@@ -269,7 +272,7 @@ def run_warm(args, cfg):
                     "4096 seeded 64x64 cavities" if cfg["weights"] == "warm64.npz" else
                     "the 64x64 network fine-tuned on seeded 257x257 cavities"),
         "config": {"workload": cfg["label"], "held_out": len(cavs), "seed": cfg.get("seed"),
-                   "eps": surrogate.EPS, "check_every": surrogate.CHECK_EVERY,
+                   "eps": eps, "check_every": surrogate.CHECK_EVERY,
                    "itc_max": cfg["itc_max"],
                    "network": {"weights": "data/" + cfg["weights"],
                                "train_samples": record.get("train_samples"),

@@ -7,8 +7,8 @@
 //             dcol[K][N]  = W[cout][K]^T . dZ[cout][N], then col2im
 // with N = batch x output pixels.  The product kernel multiplies BF16
 // operands with v_mfma_f32_16x16x16_bf16 and accumulates in FP32; one wave
-// computes a 64 x 16 block of the result (four 16 x 16 tiles sharing the
-// right operand's fragment).  Activations are FP32 in
+// computes one or four 16 x 16 tiles of the result stacked along m (sharing
+// the right operand's fragment).  Activations are FP32 in
 // channel-major layout [C][B][H][W], so a channel concatenation is a
 // contiguous stack and a layer's product output is its activation.  All
 // reductions run in a fixed order, so every call is deterministic.
@@ -51,47 +51,66 @@ struct GemmArgs {
     int M, N, K, kchunk;
 };
 
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-    // one wave per 64 x 16 block of C: up to four 16 x 16 MFMA tiles down
-    // the m axis share each B fragment, so B (the large operand) is read
-    // once per 64 rows of C instead of once per 16
+// The 16 x 16 x 16 MFMA's operand fragment of one lane: lane (r, q) holds
+// X(r, 4q + e), e = 0..3, of the 16-row slice starting at row `row` and the
+// k range [k0, k0 + 16) clipped to [.., ke), with X(i, k) = X[i si + k sk]
+// and rows >= rows zero.  When k is the contiguous axis (sk == 1) of an
+// 8-byte aligned operand and ke is a multiple of 4, the four values are
+// one 8-byte load.
+__device__ __forceinline__ v4s fragment(const bf16 *X, long si, long sk, long i, bool iok,
+                                        long k, long ke, bool vec) {
+    v4s f;
+    if (vec) {
+        uint2 w = make_uint2(0u, 0u);
+        if (iok && k < ke) w = *(const uint2 *)(X + i * si + k);
+        f[0] = (short)(w.x & 0xffffu);
+        f[1] = (short)(w.x >> 16);
+        f[2] = (short)(w.y & 0xffffu);
+        f[3] = (short)(w.y >> 16);
+        return f;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+        f[e] = (iok && k + e < ke) ? (short)X[i * si + (k + e) * sk] : (short)0;
+    return f;
+}
+
+// One wave per MT*16 x 16 block of C: MT 16 x 16 MFMA tiles down the m axis
+// share each B fragment (B, the large operand, is then read once per MT*16
+// rows of C).  The host picks MT and the k split per product so that a
+// launch has enough waves.
+template <int MT> __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int tn = (g.N + 15) >> 4, tg = (g.M + 63) >> 6;
+    const int tn = (g.N + 15) >> 4, tg = (g.M + 16 * MT - 1) / (16 * MT);
     if (wave >= tn * tg) return;   // whole waves only
-    const int mb = (wave / tn) << 6, n0 = (wave % tn) << 4;
+    const int mb = (wave / tn) * 16 * MT, n0 = (wave % tn) << 4;
     const int kb = blockIdx.z * g.kchunk, ke = min(g.K, kb + g.kchunk);
     const int r = lane & 15, q = lane >> 4;
     const long n = n0 + r;
     const bool nok = n < g.N;
-    const int nt = min(4, (g.M - mb + 15) >> 4);   // m tiles in this block (wave-uniform)
-    v4f acc[4];
+    const int nt = min(MT, (g.M - mb + 15) >> 4);   // m tiles in this block (wave-uniform)
+    const bool avec = g.as_k == 1 && (g.as_m & 3) == 0 && ((uintptr_t)g.A & 7) == 0 &&
+                      (ke & 3) == 0;
+    const bool bvec = g.bs_k == 1 && (g.bs_n & 3) == 0 && ((uintptr_t)g.B & 7) == 0 &&
+                      (ke & 3) == 0;
+    v4f acc[MT];
 #pragma unroll
-    for (int t = 0; t < 4; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = 0; t < MT; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     for (int k0 = kb; k0 < ke; k0 += 16) {
-        v4s b;
+        const long k = k0 + 4 * q;
+        const v4s b = fragment(g.B, g.bs_n, g.bs_k, n, nok, k, ke, bvec);
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const long k = k0 + 4 * q + e;
-            b[e] = (k < ke && nok) ? (short)g.B[k * g.bs_k + n * g.bs_n] : (short)0;
-        }
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < MT; t++) {
             if (t >= nt) break;
             const long m = mb + 16 * t + r;
-            const bool mok = m < g.M;
-            v4s a;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const long k = k0 + 4 * q + e;
-                a[e] = (k < ke && mok) ? (short)g.A[m * g.as_m + k * g.as_k] : (short)0;
-            }
+            const v4s a = fragment(g.A, g.as_m, g.as_k, m, m < g.M, k, ke, avec);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
         }
     }
     float *C = g.C + (long)blockIdx.z * g.M * g.ldc;
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
+    for (int t = 0; t < MT; t++) {
         if (t >= nt) break;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -111,38 +130,36 @@ __global__ void k_reduce_splits(const float *P, float *C, long ldc, int M, int N
     C[m * ldc + n] = s;
 }
 
-// col[(c k^2 + ky k + kx) N + n] = X[c][b][yo s + ky - p][xo s + kx - p], n = (b Ho + yo) Wo + xo
-__global__ void k_im2col(const float *X, bf16 *col, int C, int B, int H, int W, int ks,
-                         int stride, int pad, int Ho, int Wo) {
-    const long N = (long)B * Ho * Wo;
-    const long total = (long)C * ks * ks * N;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long)gridDim.x * blockDim.x) {
-        const long n = t % N, kk = t / N;
-        const int c = (int)(kk / (ks * ks)), rr = (int)(kk % (ks * ks));
-        const int ky = rr / ks, kx = rr % ks;
-        const int xo = (int)(n % Wo);
-        const long t2 = n / Wo;
-        const int yo = (int)(t2 % Ho), b = (int)(t2 / Ho);
+// col[(c k^2 + ky k + kx) N + n] = X[c][b][yo s + ky - p][xo s + kx - p], n = (b Ho + yo) Wo + xo.
+// Block row blockIdx.y is one row kk of col (so c, ky, kx are uniform); the
+// threads run over n.  Indices are 32-bit: N and C B H W stay below 2^31.
+__global__ __launch_bounds__(256) void k_im2col(const float *X, bf16 *col, int B, int H, int W,
+                                                int ks, int stride, int pad, int Ho, int Wo) {
+    const int N = B * Ho * Wo;
+    const int kk = blockIdx.y, c = kk / (ks * ks), rr = kk - c * ks * ks;
+    const int ky = rr / ks, kx = rr - ky * ks;
+    const float *Xc = X + (long)c * B * H * W;
+    bf16 *row = col + (long)kk * N;
+    for (int n = blockIdx.x * 256 + threadIdx.x; n < N; n += gridDim.x * 256) {
+        const int t2 = n / Wo, xo = n - t2 * Wo;
+        const int b = t2 / Ho, yo = t2 - b * Ho;
         const int y = yo * stride + ky - pad, x = xo * stride + kx - pad;
         float v = 0.0f;
-        if (y >= 0 && y < H && x >= 0 && x < W) v = X[(((long)c * B + b) * H + y) * W + x];
-        col[t] = to_bf16(v);
+        if (y >= 0 && y < H && x >= 0 && x < W) v = Xc[(b * H + y) * W + x];
+        row[n] = to_bf16(v);
     }
 }
 
 // dX[c][b][y][x] (+)= sum of the dcol entries that im2col read from (c, b, y, x)
-__global__ void k_col2im(const float *dcol, float *dX, int C, int B, int H, int W, int ks,
-                         int stride, int pad, int Ho, int Wo, int accumulate) {
-    const long N = (long)B * Ho * Wo;
-    const long total = (long)C * B * H * W;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long)gridDim.x * blockDim.x) {
-        const int x = (int)(t % W);
-        const long t2 = t / W;
-        const int y = (int)(t2 % H);
-        const long t3 = t2 / H;
-        const int b = (int)(t3 % B), c = (int)(t3 / B);
+__global__ __launch_bounds__(256) void k_col2im(const float *dcol, float *dX, int C, int B, int H,
+                                                int W, int ks, int stride, int pad, int Ho, int Wo,
+                                                int accumulate) {
+    const int N = B * Ho * Wo;
+    const int total = C * B * H * W;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < total; t += gridDim.x * 256) {
+        const int t2 = t / W, x = t - t2 * W;
+        const int t3 = t2 / H, y = t2 - t3 * H;
+        const int c = t3 / B, b = t3 - c * B;
         float s = 0.0f;
         for (int ky = 0; ky < ks; ky++) {
             const int yy = y + pad - ky;
@@ -154,7 +171,7 @@ __global__ void k_col2im(const float *dcol, float *dX, int C, int B, int H, int 
                 if (xx < 0 || xx % stride) continue;
                 const int xo = xx / stride;
                 if (xo >= Wo) continue;
-                s += dcol[((long)c * ks * ks + ky * ks + kx) * N + ((long)b * Ho + yo) * Wo + xo];
+                s += dcol[(long)((c * ks + ky) * ks + kx) * N + (b * Ho + yo) * Wo + xo];
             }
         }
         dX[t] = accumulate ? dX[t] + s : s;
@@ -359,24 +376,45 @@ uint64_t splitmix(uint64_t &s) {
     return z ^ (z >> 31);
 }
 
+// partial products of split-k products (splits x M x N floats) and the
+// bias-gradient partial sums
+constexpr long kPartFloats = 64L * 64 * 128 * 9;
+
 hipError_t gemm(lbm_nn n, int M, int N, int K, const bf16 *A, long as_m, long as_k,
                 const bf16 *B, long bs_k, long bs_n, float *C, long ldc) {
-    // split the k range when the output is small and k is long
-    const long tiles = (long)((M + 15) / 16) * ((N + 15) / 16);
-    const long blocks = (long)((M + 63) / 64) * ((N + 15) / 16);   // one wave each
-    int splits = 1;
-    if (tiles < 1024 && K >= 4096) splits = (int)std::min<long>(64, std::max<long>(1, K / 2048));
+    // Waves per launch: one per MT*16 x 16 block of C and k range.  Split
+    // the k range (at least 256 long, into the partials buffer) until a
+    // launch has about kWaves waves; take four tiles per wave unless one
+    // tile per wave gives more waves.
+    constexpr long kWaves = 2048;
+    const long tn = (N + 15) / 16, mn = (long)M * N;
+    auto plan = [&](int mt, int &splits) {
+        const long blocks = ((M + 16L * mt - 1) / (16L * mt)) * tn;
+        const long want = (kWaves + blocks - 1) / blocks;
+        splits = (int)std::max<long>(
+            1, std::min<long>({want, std::max<long>(1, K / 256), 64L, kPartFloats / mn}));
+        return blocks * splits;
+    };
+    int s4 = 1, s1 = 1, mt = 4, splits;
+    const long w4 = plan(4, s4);
+    splits = s4;
+    if (w4 < kWaves && M > 16 && plan(1, s1) > w4) {
+        mt = 1;
+        splits = s1;
+    }
     const int kchunk = ((K + splits - 1) / splits + 15) / 16 * 16;
     splits = (K + kchunk - 1) / kchunk;
     nn::GemmArgs g{A, as_m, as_k, B, bs_k, bs_n, splits > 1 ? n->part : C,
                    splits > 1 ? (long)N : ldc, M, N, K, kchunk};
+    const long blocks = ((M + 16L * mt - 1) / (16L * mt)) * tn;
     dim3 grid((unsigned)((blocks + 3) / 4), 1, (unsigned)splits);
-    hipLaunchKernelGGL(nn::k_gemm, grid, dim3(256), 0, n->s, g);
-    if (splits > 1) {
-        const long mn = (long)M * N;
+    if (mt == 4)
+        hipLaunchKernelGGL(nn::k_gemm<4>, grid, dim3(256), 0, n->s, g);
+    else
+        hipLaunchKernelGGL(nn::k_gemm<1>, grid, dim3(256), 0, n->s, g);
+    if (splits > 1)
         hipLaunchKernelGGL(nn::k_reduce_splits, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0,
                            n->s, (const float *)n->part, C, ldc, M, N, splits);
-    }
     return hipGetLastError();
 }
 
@@ -384,8 +422,8 @@ hipError_t gemm(lbm_nn n, int M, int N, int K, const bf16 *A, long as_m, long as
 hipError_t conv_fwd(lbm_nn n, const ConvLayer &l, int B, const float *X, int H, float *Y) {
     const int Ho = H / l.stride, pad = l.ks / 2;
     const long N = (long)B * Ho * Ho, K = (long)l.cin * l.ks * l.ks;
-    hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(K * N)), dim3(256), 0, n->s, X, n->col,
-                       l.cin, B, H, H, l.ks, l.stride, pad, Ho, Ho);
+    hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(N), (unsigned)K), dim3(256), 0, n->s, X,
+                       n->col, B, H, H, l.ks, l.stride, pad, Ho, Ho);
     hipError_t e = gemm(n, l.cout, (int)N, (int)K, n->wb + l.woff, K, 1, n->col, N, 1, Y, N);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(nn::k_bias_act, dim3(nn::grid_for((long)l.cout * N)), dim3(256), 0, n->s,
@@ -411,8 +449,8 @@ hipError_t conv_bwd(lbm_nn n, const ConvLayer &l, int B, const float *X, int H, 
                        (const float *)n->part, n->g + l.boff, l.cout, P);
     hipLaunchKernelGGL(nn::k_to_bf16, dim3(nn::grid_for(l.cout * N)), dim3(256), 0, n->s,
                        (const float *)dA, n->dzb, (long)l.cout * N);
-    hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(K * N)), dim3(256), 0, n->s, X, n->col,
-                       l.cin, B, H, H, l.ks, l.stride, pad, Ho, Ho);
+    hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(N), (unsigned)K), dim3(256), 0, n->s, X,
+                       n->col, B, H, H, l.ks, l.stride, pad, Ho, Ho);
     // dW[cout][K] = dZ[cout][N] . col[K][N]^T
     hipError_t e = gemm(n, l.cout, (int)K, (int)N, n->dzb, N, 1, n->col, 1, N, n->g + l.woff, K);
     if (e != hipSuccess || !dX) return e;
@@ -567,7 +605,7 @@ int32_t lbm_nn_create(int32_t grid, int32_t batch_max, uint64_t seed, int32_t de
     A(n->da4, 64 * B * P8); A(n->da5, 64 * B * P4); A(n->da6, 32 * B * P2); A(n->da7, 16 * B * P);
     A(n->dy, 3 * B * P);
     A(n->col, colmax); A(n->dzb, zmax); A(n->dcol, colmax);
-    A(n->part, 64L * 64 * 128 * 9); A(n->red, nn::kMseBlocks);
+    A(n->part, kPartFloats); A(n->red, nn::kMseBlocks);
     A(n->stage, 4 * B * P); A(n->tgt, 3 * B * P);
     if (rc) {
         release(n);

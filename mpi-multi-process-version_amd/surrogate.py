@@ -216,12 +216,13 @@ def warm_fields(net, cavity, s):
     return lbmnn.extrude(u, v, T, s.n)
 
 
-def traced_run(cavity, net=None, eps=EPS, itc_max=600000):
+def traced_run(cavity, net=None, eps=EPS, itc_max=600000, log=None):
     """One cavity to the stopping rule (the loop of lbm_run(): check every
     CHECK_EVERY steps, stop when errU and errT are both <= eps), from rest
     or from the network's prediction, recording after every check the step
     count, the wall time since the start (inference and lbm_init_fields
-    included) and the hot-wall Nusselt number."""
+    included) and the hot-wall Nusselt number; log (if given) receives a
+    line every 50 checks."""
     c = dict(cavity)
     s = lbmhip.Solver(c.pop("dim", 2), c.pop("n"), c.pop("Ra"), Pr=c.pop("Pr"),
                       stretch=c.pop("stretch"), epsU=eps, epsT=eps, check_every=CHECK_EVERY,
@@ -243,6 +244,9 @@ def traced_run(cavity, net=None, eps=EPS, itc_max=600000):
             break
         nu = s.nusselt()[0]
         trace.append((s.itc, time.perf_counter() - t0, nu))
+        if log and len(trace) % 50 == 0:
+            log(json.dumps(dict(stage="run", start="cold" if net is None else "warm",
+                                itc=s.itc, errU=eu, errT=et, nu=nu)))
         if eu <= eps and et <= eps:
             status = lbmhip.LBM_OK
             break
@@ -275,8 +279,8 @@ def compare(net, cavities, eps=EPS, itc_max=600000, tol=1e-3, log=None):
     net.forward(np.zeros((1, 4, net.grid, net.grid), np.float32))   # load the kernels once
     rows = []
     for i, cav in enumerate(cavities):
-        cold = traced_run(cav, None, eps, itc_max)
-        warm = traced_run(cav, net, eps, itc_max)
+        cold = traced_run(cav, None, eps, itc_max, log)
+        warm = traced_run(cav, net, eps, itc_max, log)
         row = dict(Ra=cav["Ra"], Pr=cav["Pr"], stretch=cav["stretch"],
                    cold_status=cold["status"], warm_status=warm["status"],
                    cold_itc=int(cold["itc"]), warm_itc=int(warm["itc"]),
