@@ -416,3 +416,35 @@ def test_graph_replays_match_direct_launches(hip, dim, n, prec, launch):
         assert np.array_equal(x, y)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("dim,n,prec,launch,steps,tol", [
+    (2, 65, "fp64", None, 400, 1e-11),
+    (3, 16, "fp64", None, 150, 1e-11),
+    (2, 65, "fp32", (64, 4, 1, 0), 400, FP32_TOL),
+    (2, 66, "fp32", (64, 1, 1, 1), 400, FP32_TOL),
+    (3, 32, "fp32", (64, 4, 2, 1), 300, FP32_TOL),
+    (3, 32, "fp32", (64, 4, 1, 0), 300, FP32_TOL),
+])
+def test_trajectory_keeps_the_cavity_symmetries(hip, dim, n, prec, launch, steps, tol):
+    """The cavity's mirror and half-turn symmetries (cavity_symmetry.py;
+    walls at +-1/2 so that the half turns are exact) hold for the GPU
+    trajectory to rounding: FP64 to 1e-11, the FP32 paths (the plain
+    kernel, the 2D y-march and the 3D z-march) to the FP32 bar.  A lane,
+    tile or halo index that is wrong on one side of a tile or of the grid
+    breaks them."""
+    from cavity_symmetry import cavity_asymmetry
+    p = hip.PREC_FP64 if prec == "fp64" else hip.PREC_FP32_SHIFTED
+    s = hip.Solver(dim, n, 1e5, Thot=0.5, Tcold=-0.5, stretch=2.0, precision=p)
+    if launch:
+        s.set_launch(launch[0], launch[1], remap=launch[2], variant=launch[3])
+    print(f"\n[symmetry] {dim}D n={n} {prec} launch={launch}", flush=True)
+    s.init_rest()
+    s.step(steps)
+    f = s.fields()
+    s.close()
+    assert np.abs(f[1] if dim == 2 else f[3]).max() > 1e-4
+    a = cavity_asymmetry(f, dim, t_sum=0.0)
+    print(f"  {a}", flush=True)
+    for name, val in a.items():
+        assert val < tol, (name, val)
