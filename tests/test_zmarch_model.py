@@ -87,7 +87,7 @@ def window(k, n):
 
 def planes_per_group(n, by):
     cols = ((n + 63) // 64) * ((n + by - 1) // by)
-    nzc = max(1, min((2048 + cols - 1) // cols, max(1, n // 4)))
+    nzc = max(1, min((3072 + cols - 1) // cols, max(1, n // 4)))
     return (n + nzc - 1) // nzc
 
 
