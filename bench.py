@@ -159,8 +159,9 @@ def run_steady(args, cfg):
     """The reference's own case run to steady state: 2D 129^2, Ra = 1e4,
     Pr = 0.71, from the reference IC to both residuals <= 1e-8 in FP64,
     checking every 2000 steps, then the FP32 shifted path for the same number
-    of steps with its Nu against the FP64 one.  The value is the FP64 wall
-    time from lbm_create to the converged check."""
+    of steps with its Nu against the FP64 one, then the learned warm start
+    against the cold start (both FP64, same rule).  The value is the FP64
+    cold wall time from lbm_create to the converged check."""
     import lbmhip
     runs = {}
     itc_max = 2000000
@@ -201,6 +202,25 @@ def run_steady(args, cfg):
                              "bounds it); check intervals replay as HIP graphs"},
         "check": runs,
     }
+    # the learned warm start on the same case, FP64 to the same rule
+    import surrogate
+    weights = os.path.join(os.path.dirname(surrogate.WEIGHTS), "warm257.npz")
+    net = surrogate.load_net(weights)
+    cav = dict(n=cfg["n"], Ra=cfg["Ra"], Pr=0.71, stretch=3.2, precision=lbmhip.PREC_FP64)
+    rows, sm = surrogate.compare(net, [cav], eps=1e-8, itc_max=2000000)
+    net.close()
+    w = rows[0]
+    out["warm_start"] = {
+        "weights": "data/warm257.npz", "precision": "f64", "eps": 1e-8,
+        "cold_itc": w["cold_itc"], "warm_itc": w["warm_itc"],
+        "cold_s": round(w["cold_s"], 4), "warm_s": round(w["warm_s"], 4),
+        "stop_wall_ratio": round(w["cold_s"] / w["warm_s"], 3),
+        "nu_cold": w["cold_nu"], "nu_warm": w["warm_nu"], "nu_rel_dev": w.get("nu_rel_dev"),
+        "nu_band_itc": [w.get("cold_band_itc"), w.get("warm_band_itc")],
+        "nu_band_s": [w.get("cold_band_s"), w.get("warm_band_s")],
+        "nu_band_wall_ratio": sm.get("band_wall_ratio"), "fallback": w.get("fallback"),
+        "note": "traced runs (a check and a Nusselt number every 2000 steps, on the host); "
+                "the warm time includes inference, resampling and lbm_init_fields"}
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import lbm_oracle

@@ -129,14 +129,14 @@ def predict(net, x):
 
 
 def generate(count, seed, eps=EPS, itc_max=600000, per_batch=256, keep_tol=1e-4, n=GRID,
-             log=print):
+             stretch=STRETCH, log=print):
     """Steady fields of `count` seeded cavities on n x n grids, as network
     inputs and targets on the network's G x G nodes (sampled there from a
     larger grid, see net_inputs).  A cavity is kept if it reached the
     stopping rule, or ran to itc_max with errU and errT below keep_tol (a
     slowly settling cavity, already steady to that tolerance); diverged
     cavities are dropped.  Returns (x, y, record)."""
-    cavs = sample_cavities(count, seed, n=n)
+    cavs = sample_cavities(count, seed, n=n, stretch=stretch)
     xs, ys, keep, itcs = [], [], [], []
     t0 = time.perf_counter()
     for s in range(0, count, per_batch):
@@ -219,14 +219,16 @@ def warm_fields(net, cavity, s):
 def traced_run(cavity, net=None, eps=EPS, itc_max=600000, log=None):
     """One cavity to the stopping rule (the loop of lbm_run(): check every
     CHECK_EVERY steps, stop when errU and errT are both <= eps), from rest
-    or from the network's prediction, recording after every check the step
+    or from the network's prediction (FP32 shifted storage unless the
+    cavity names a `precision`), recording after every check the step
     count, the wall time since the start (inference and lbm_init_fields
     included) and the hot-wall Nusselt number; log (if given) receives a
     line every 50 checks."""
     c = dict(cavity)
     s = lbmhip.Solver(c.pop("dim", 2), c.pop("n"), c.pop("Ra"), Pr=c.pop("Pr"),
                       stretch=c.pop("stretch"), epsU=eps, epsT=eps, check_every=CHECK_EVERY,
-                      itc_max=itc_max)
+                      itc_max=itc_max,
+                      precision=c.pop("precision", lbmhip.PREC_FP32_SHIFTED))
     if s.dim == 3:
         s.set_launch(64, 4, remap=2, variant=1)   # the LDS z-march
     t0 = time.perf_counter()
@@ -354,6 +356,8 @@ def main(argv=None):
     g.add_argument("--itc-max", type=int, default=600000)
     g.add_argument("--per-batch", type=int, default=256)
     g.add_argument("--keep-tol", type=float, default=1e-4)
+    g.add_argument("--stretch", type=float, nargs=2, default=list(STRETCH),
+                   help="range of the erf stretch factor")
     t = sub.add_parser("train")
     t.add_argument("--out", default=WEIGHTS)
     t.add_argument("--data", nargs="*", default=None,
@@ -378,8 +382,8 @@ def main(argv=None):
     log = lambda s: print(s, flush=True)
     if a.cmd == "generate":
         x, y, rec = generate(a.count, a.seed, itc_max=a.itc_max, per_batch=a.per_batch,
-                             keep_tol=a.keep_tol, n=a.n, log=log)
-        rec.update(n=a.n)
+                             keep_tol=a.keep_tol, n=a.n, stretch=tuple(a.stretch), log=log)
+        rec.update(n=a.n, stretch=a.stretch)
         save_data(a.out, x, y, rec)
         log(json.dumps(dict(stage="saved", path=a.out, **rec)))
     elif a.cmd == "train":
