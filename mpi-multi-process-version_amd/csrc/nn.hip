@@ -150,6 +150,34 @@ __global__ __launch_bounds__(256) void k_im2col(const float *X, bf16 *col, int B
     }
 }
 
+// k_im2col for Wo % 8 == 0: a thread writes eight consecutive n (the same
+// image row, xo = 8m .. 8m+7) as one 16-byte store; the same values.
+__global__ __launch_bounds__(256) void k_im2col8(const float *X, bf16 *col, int B, int H, int W,
+                                                 int ks, int stride, int pad, int Ho, int Wo) {
+    const int N = B * Ho * Wo;
+    const int kk = blockIdx.y, c = kk / (ks * ks), rr = kk - c * ks * ks;
+    const int ky = rr / ks, kx = rr - ky * ks;
+    const float *Xc = X + (long)c * B * H * W;
+    bf16 *row = col + (long)kk * N;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < (N >> 3); t += gridDim.x * 256) {
+        const int n = t << 3;
+        const int t2 = n / Wo, xo = n - t2 * Wo;
+        const int b = t2 / Ho, yo = t2 - b * Ho;
+        const int y = yo * stride + ky - pad;
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        if (y >= 0 && y < H) {
+            const float *xr = Xc + (b * H + y) * W;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int x = (xo + e) * stride + kx - pad;
+                const float v = (x >= 0 && x < W) ? xr[x] : 0.0f;
+                w[e >> 1] |= (uint32_t)to_bf16(v) << (16 * (e & 1));
+            }
+        }
+        *reinterpret_cast<uint4 *>(row + n) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
 // dX[c][b][y][x] (+)= sum of the dcol entries that im2col read from (c, b, y, x)
 __global__ __launch_bounds__(256) void k_col2im(const float *dcol, float *dX, int C, int B, int H,
                                                 int W, int ks, int stride, int pad, int Ho, int Wo,
@@ -418,12 +446,22 @@ hipError_t gemm(lbm_nn n, int M, int N, int K, const bf16 *A, long as_m, long as
     return hipGetLastError();
 }
 
+// col of the K x N im2col matrix of X ([C][B][H][H], Ho x Ho outputs)
+void im2col(lbm_nn n, const float *X, int B, int H, int ks, int stride, int pad, int Ho, long K) {
+    const long N = (long)B * Ho * Ho;
+    if (Ho % 8 == 0)
+        hipLaunchKernelGGL(nn::k_im2col8, dim3(nn::grid_for(N / 8), (unsigned)K), dim3(256), 0,
+                           n->s, X, n->col, B, H, H, ks, stride, pad, Ho, Ho);
+    else
+        hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(N), (unsigned)K), dim3(256), 0, n->s,
+                           X, n->col, B, H, H, ks, stride, pad, Ho, Ho);
+}
+
 // forward of one convolution: Y[cout][B][Ho][Wo] = act(W * X + b)
 hipError_t conv_fwd(lbm_nn n, const ConvLayer &l, int B, const float *X, int H, float *Y) {
     const int Ho = H / l.stride, pad = l.ks / 2;
     const long N = (long)B * Ho * Ho, K = (long)l.cin * l.ks * l.ks;
-    hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(N), (unsigned)K), dim3(256), 0, n->s, X,
-                       n->col, B, H, H, l.ks, l.stride, pad, Ho, Ho);
+    im2col(n, X, B, H, l.ks, l.stride, pad, Ho, K);
     hipError_t e = gemm(n, l.cout, (int)N, (int)K, n->wb + l.woff, K, 1, n->col, N, 1, Y, N);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(nn::k_bias_act, dim3(nn::grid_for((long)l.cout * N)), dim3(256), 0, n->s,
@@ -449,8 +487,7 @@ hipError_t conv_bwd(lbm_nn n, const ConvLayer &l, int B, const float *X, int H, 
                        (const float *)n->part, n->g + l.boff, l.cout, P);
     hipLaunchKernelGGL(nn::k_to_bf16, dim3(nn::grid_for(l.cout * N)), dim3(256), 0, n->s,
                        (const float *)dA, n->dzb, (long)l.cout * N);
-    hipLaunchKernelGGL(nn::k_im2col, dim3(nn::grid_for(N), (unsigned)K), dim3(256), 0, n->s, X,
-                       n->col, B, H, H, l.ks, l.stride, pad, Ho, Ho);
+    im2col(n, X, B, H, l.ks, l.stride, pad, Ho, K);
     // dW[cout][K] = dZ[cout][N] . col[K][N]^T
     hipError_t e = gemm(n, l.cout, (int)K, (int)N, n->dzb, N, 1, n->col, 1, N, n->g + l.woff, K);
     if (e != hipSuccess || !dX) return e;
