@@ -342,10 +342,7 @@ __device__ __forceinline__ bool zm_tile(int P, const StepArgs<float> &a, int nzc
 }
 
 template <int BY, bool MACRO, bool NT>
-#ifndef LBM_ZM_WAVES
-#define LBM_ZM_WAVES 1
-#endif
-__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(LBM_ZM_WAVES))) void k_step_zm(StepArgs<float> a, int kc) {
+__global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) {
     using L = ZmLayout<BY>;
     using FL = Flow<3>;
     using HT = Heat<3>;
