@@ -65,7 +65,8 @@ typedef struct lbm_params {
     int32_t precision;    /* LBM_PREC_*                                      */
     double  epsU, epsT;   /* convergence thresholds of lbm_run               */
     int32_t check_every;  /* residual window in steps (reference: 2000)      */
-    int32_t device;       /* HIP device ordinal                              */
+    int32_t device;       /* HIP device ordinal (>= 0; there is no CPU
+                             backend)                                        */
     int64_t itc_max;      /* step limit of lbm_run                           */
 } lbm_params;
 

@@ -315,6 +315,9 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
     if (p.precision != LBM_PREC_FP32_SHIFTED && p.precision != LBM_PREC_FP64)
         return fail(LBM_ERR_ARG, "unknown precision");
     if (p.check_every < 1 || p.itc_max < 0) return fail(LBM_ERR_ARG, "check_every/itc_max");
+    if (p.device < 0)
+        return fail(LBM_ERR_ARG, "device must be a HIP device ordinal: this library has no CPU "
+                                 "backend");
 
     lbm_ctx *c = new (std::nothrow) lbm_ctx();
     if (!c) return fail(LBM_ERR_NOMEM, "host allocation");

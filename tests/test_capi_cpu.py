@@ -38,7 +38,7 @@ def test_params_struct_layout(hip):
 @pytest.mark.parametrize("kw", [dict(dim=4), dict(n=4), dict(Ra=-1.0),
                                 dict(Ma=1e-12), dict(Thot=0.0, Tcold=0.0),
                                 dict(precision=7), dict(grid_kind=9),
-                                dict(check_every=0)])
+                                dict(check_every=0), dict(device=-1)])
 def test_create_rejects_bad_parameters(hip, kw):
     args = dict(dim=2, n=17, Ra=1e4)
     args.update(kw)
