@@ -34,6 +34,7 @@ struct lbm_batch_ctx {
     int bx = 64, by = 4;
     int variant = 0;   // 0: one node per thread; 1: y-march strips
     int kc = 0;        // variant 1: rows per chunk
+    double *h_red = nullptr;   // pinned host memory: 4 residual sums per cavity
 };
 
 namespace {
@@ -58,6 +59,7 @@ void release(lbm_batch b) {
     if (b->d_tiles) (void)hipFree(b->d_tiles);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->h_red) (void)hipHostFree(b->h_red);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -204,6 +206,12 @@ int32_t lbm_batch_create(const lbm_params *params, int32_t count, lbm_batch *out
     }
     b->active.assign(count, 0);
     b->base_cur.assign(count, 0);
+    e = hipHostMalloc((void **)&b->h_red, sizeof(double) * 4 * (size_t)count);
+    if (e != hipSuccess) {
+        b->h_red = nullptr;
+        release(b);
+        return hip_err(e, "batch residual buffer");
+    }
     std::vector<lbm::StepArgs<float>> args(2 * (size_t)count);
     for (int32_t c = 0; c < count; c++)
         for (int p = 0; p < 2; p++) {
@@ -305,13 +313,27 @@ int32_t lbm_batch_run(lbm_batch b, int64_t *itc, double *errU, double *errT, int
         }
         if (n == INT64_MAX) break;   // no active cavity left
         if ((rc = batch_steps(b, n))) return rc;
+        // lbm_check of every cavity due for one, with one wait: the
+        // residual kernels and the copies of their sums are queued on the
+        // batch's stream, then read together
+        bool due = false;
+        for (size_t c = 0; c < nc; c++) {
+            if (!b->active[c] || b->cav[c]->itc % every != 0) continue;
+            hipError_t e = lbm::launch_residual(*b->cav[c]);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b->h_red + 4 * c, b->cav[c]->d_red, 4 * sizeof(double),
+                                   hipMemcpyDeviceToHost, b->stream);
+            if (e != hipSuccess) return hip_err(e, "batch residual");
+            due = true;
+        }
+        if (due) BATCH_TRY(hipStreamSynchronize(b->stream));
         bool changed = false;
         for (size_t c = 0; c < nc; c++) {
             if (!b->active[c]) continue;
             lbm_ctx &h = *b->cav[c];
             bool done = false;
             if (h.itc % every == 0) {
-                rc = lbm_check(&h, &eu[c], &et[c]);
+                rc = lbm::residual_ratios(b->h_red + 4 * c, &eu[c], &et[c]);
                 if (rc == LBM_ERR_DIVERGED) {
                     st[c] = LBM_ERR_DIVERGED;
                     done = true;

@@ -287,6 +287,23 @@ namespace lbm {
 int32_t set_error(int32_t code, const std::string &msg) { return fail(code, msg); }
 }  // namespace lbm
 
+namespace lbm {
+
+int32_t residual_ratios(const double r[4], double *errU, double *errT) {
+    auto ratio = [](double num, double den) {
+        if (num == 0.0 && den == 0.0) return 0.0;
+        return std::sqrt(num) / std::sqrt(den);
+    };
+    const double eu = ratio(r[0], r[1]), et = ratio(r[2], r[3]);
+    if (errU) *errU = eu;
+    if (errT) *errT = et;
+    if (!std::isfinite(eu) || !std::isfinite(et))
+        return set_error(LBM_ERR_DIVERGED, "non-finite residual");
+    return LBM_OK;
+}
+
+}  // namespace lbm
+
 extern "C" {
 
 int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
@@ -509,16 +526,7 @@ int32_t lbm_check(lbm_handle h, double *errU, double *errT) {
     double r[4];
     HIP_TRY(hipMemcpyAsync(r, h->d_red, sizeof r, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    auto ratio = [](double num, double den) {
-        if (num == 0.0 && den == 0.0) return 0.0;
-        return std::sqrt(num) / std::sqrt(den);
-    };
-    double eu = ratio(r[0], r[1]), et = ratio(r[2], r[3]);
-    if (errU) *errU = eu;
-    if (errT) *errT = et;
-    if (!std::isfinite(eu) || !std::isfinite(et))
-        return fail(LBM_ERR_DIVERGED, "non-finite residual");
-    return LBM_OK;
+    return lbm::residual_ratios(r, errU, errT);
 }
 
 int32_t lbm_nusselt(lbm_handle h, double *nu_hot, double *nu_cold) {

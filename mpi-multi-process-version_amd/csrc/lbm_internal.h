@@ -88,6 +88,9 @@ hipError_t launch_prologue_f64(int dim, const StepArgs<double> &a,
 // Auxiliary kernels (lbm_aux.hip), all FP64.
 hipError_t launch_init_rest(const Ctx &c);
 hipError_t launch_residual(const Ctx &c);   // results: d_red[0..3]
+// errU and errT from the four sums of launch_residual (lbm_check's
+// arithmetic); LBM_ERR_DIVERGED if either is not finite.
+int32_t residual_ratios(const double r[4], double *errU, double *errT);
 hipError_t launch_nusselt(const Ctx &c);    // results: d_red[4..5]
 
 // Batched step (lbm_step_f32.hip): one launch advances every cavity listed
