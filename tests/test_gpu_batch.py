@@ -156,3 +156,45 @@ def test_batch_autotune_restores_state(hip):
     cav, nodes = b.active()
     assert cav == len(CAVITIES) and nodes == sum(c["n"] ** 2 for c in CAVITIES)
     b.close()
+
+
+def test_batch_mall_groups_keep_every_cavity_exact(hip):
+    """A batch whose step traffic exceeds half the 256 MB MALL runs in groups
+    of whole cavities, each advancing up to 64 steps before the next group
+    starts.  Every cavity's arithmetic is unchanged: the same cavities split
+    into batches small enough to run ungrouped (the same kernel, launch
+    shape and ordinary stores) must give the same fields to the bit, over
+    full and partial group chunks and an odd count; and each cavity agrees
+    with its single-cavity run to FP32 rounding."""
+    cav = [dict(n=513 if i % 3 else 257, Ra=10 ** (3 + 3 * i / 13), Pr=0.7 + 0.4 * i,
+                stretch=1.0 + 0.1 * i) for i in range(14)]
+    print(f"\n[batch MALL groups] {len(cav)} cavities, "
+          f"{sum(c['n'] ** 2 for c in cav) / 1e6:.2f} M nodes", flush=True)
+    split = [cav[k:k + 3] for k in range(0, len(cav), 3)]
+    # each part is below the 128 MB group limit of step traffic (112 B per
+    # node), the whole batch well above it
+    assert all(sum(c["n"] ** 2 for c in h) * 112 < 128 * 2 ** 20 for h in split)
+    assert sum(c["n"] ** 2 for c in cav) * 112 > 256 * 2 ** 20
+    a = hip.Batch(cav)
+    parts = [hip.Batch(h) for h in split]
+    for x in [a] + parts:
+        x.set_launch(128, 4)
+        x.init_rest()
+        x.step(70)
+        x.step(7)
+    for i in range(len(cav)):
+        part, j = parts[i // 3], i % 3
+        assert a[i].itc == part[j].itc == 77
+        for x, y in zip(a[i].fields(), part[j].fields()):
+            assert np.array_equal(x, y), i
+    for i in (0, 1, 13):
+        s = _single(hip, cav[i])
+        s.set_launch(128, 4, remap=0)
+        s.init_rest()
+        s.step(77)
+        errs = [_rel(x, y) for x, y in zip(a[i].fields(), s.fields()) if np.abs(y).max() > 0]
+        print(f"  cavity {i} n={cav[i]['n']}: max rel diff vs single {max(errs):.3g}", flush=True)
+        assert max(errs) <= 1e-6, (i, errs)
+        s.close()
+    for x in [a] + parts:
+        x.close()
