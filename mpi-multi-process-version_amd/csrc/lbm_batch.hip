@@ -38,6 +38,7 @@ struct lbm_batch_ctx {
     // variant 0: tile-table offsets of the MALL groups (whole cavities in
     // table order), groups[g] .. groups[g + 1]; one group when all fit
     std::vector<int> groups;
+    std::vector<double> group_nodes;   // nodes of each group
 };
 
 namespace {
@@ -104,6 +105,7 @@ int32_t rebuild(lbm_batch b) {
                      [&](int x, int y) { return b->cav[x]->n > b->cav[y]->n; });
     std::vector<int4> tiles;
     b->groups.assign(1, 0);
+    b->group_nodes.clear();
     double gnodes = 0.0;
     if (b->variant == 1) {
         double strip_rows = 0.0;
@@ -127,6 +129,7 @@ int32_t rebuild(lbm_batch b) {
             const double nodes = (double)h.n * h.n;
             if (gnodes > 0.0 && (gnodes + nodes) * kStepBytesPerNode > kGroupBytes) {
                 b->groups.push_back((int)tiles.size());
+                b->group_nodes.push_back(gnodes);
                 gnodes = 0.0;
             }
             gnodes += nodes;
@@ -154,6 +157,7 @@ int32_t rebuild(lbm_batch b) {
                             hipMemcpyHostToDevice));
     b->ntiles = (int)tiles.size();
     b->groups.push_back(b->ntiles);
+    b->group_nodes.push_back(gnodes);
     b->parity = 0;
     b->dirty = false;
     return LBM_OK;
@@ -192,14 +196,17 @@ int32_t batch_steps(lbm_batch b, int64_t nsteps) {
         // kGroupDepth steps before the next group starts: its steps after
         // the first read what the previous one wrote from the MALL (stores
         // stay ordinary, so that they are kept there) instead of from HBM.
+        // A group of one cavity too large for the MALL streams its stores.
         for (int64_t s0 = 0; s0 < nsteps; s0 += kGroupDepth) {
             const int64_t d = std::min(kGroupDepth, nsteps - s0);
-            for (int g = 0; g < ng; g++)
+            for (int g = 0; g < ng; g++) {
+                const bool nt = lbm::stream_stores(b->group_nodes[g] * kStepBytesPerNode);
                 for (int64_t s = 0; s < d; s++) {
                     hipError_t e = launch(g, g + 1, b->parity ^ (int)(s & 1),
-                                          s0 + s == nsteps - 1, false);
+                                          s0 + s == nsteps - 1, nt);
                     if (e != hipSuccess) return hip_err(e, "batched step launch");
                 }
+            }
             b->parity ^= (int)(d & 1);
         }
     }
