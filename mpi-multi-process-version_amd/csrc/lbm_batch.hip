@@ -31,7 +31,12 @@ struct lbm_batch_ctx {
     int ntiles = 0, tiles_cap = 0;
     int parity = 0;   // launches since the table was built, mod 2
     bool dirty = true;
-    int bx = 64, by = 4;
+    // default block (lbm_batch_set_launch and lbm_batch_autotune change
+    // it): 64 x 8 beat 64 x 4, 64 x 2, 32 x 4 and 32 x 2 on batches of 256
+    // cavities of 64^2 (24.4 against 26.6 us per step), 128 of 129^2
+    // (73.8 against 76.9) and 64 of 257^2 (137.8 against 145.7; 128 x 4:
+    // 132.4)
+    int bx = 64, by = 8;
     int variant = 0;   // 0: one node per thread; 1: y-march strips
     int kc = 0;        // variant 1: rows per chunk
     double *h_red = nullptr;   // pinned host memory: 4 residual sums per cavity
