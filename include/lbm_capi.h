@@ -187,7 +187,10 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes);
  *      3D: a 64 x by column of nodes in z; bx = 64, by = 2, 4, 6 or 8,
  *          n % 4 == 0 and n <= 544;
  *      2D: one wave per 64 columns, in y; bx = 64, by = 1.
- *    The variants read the same stencil values and agree to FP32 rounding. */
+ *    The variants read the same stencil values and agree to FP32 rounding.
+ * Defaults set by lbm_create: 3D FP32 with n % 4 == 0 and n <= 544: variant
+ * 1, 64 x 4, remap 2; 2D with n <= 384: 32 x 2, remap 1; otherwise 64 x 4,
+ * remap 1, variant 0. */
 typedef struct lbm_launch {
     int32_t bx, by, remap, occupancy, variant;
 } lbm_launch;

@@ -354,6 +354,15 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
         c->shape.bx = 32;
         c->shape.by = 2;
     }
+    // 3D FP32: the LDS z-march (variant 1, 64 x 4, XCD tile bands) wherever
+    // its preconditions hold (those of lbm_set_launch); at 256^3 it takes
+    // 0.72 ms per step against 1.10 ms for the plain kernel
+    if (p.dim == 3 && p.precision == LBM_PREC_FP32_SHIFTED && c->n % 4 == 0 && c->n <= 544) {
+        c->shape.bx = 64;
+        c->shape.by = 4;
+        c->shape.remap = 2;
+        c->shape.variant = 1;
+    }
     c->plane = (c->nn + 63) / 64 * 64;
     c->fp64 = p.precision == LBM_PREC_FP64;
     build_grid(*c);
