@@ -449,3 +449,22 @@ def test_trajectory_keeps_the_cavity_symmetries(hip, dim, n, prec, launch, steps
     print(f"  {a}", flush=True)
     for name, val in a.items():
         assert val < tol, (name, val)
+
+
+@pytest.mark.parametrize("dim,n,prec,want", [
+    (3, 32, "fp32", dict(bx=64, by=4, remap=2, variant=1)),
+    (3, 30, "fp32", dict(bx=64, by=4, remap=1, variant=0)),
+    (3, 32, "fp64", dict(bx=64, by=4, remap=1, variant=0)),
+    (2, 129, "fp32", dict(bx=32, by=2, remap=1, variant=0)),
+    (2, 129, "fp64", dict(bx=32, by=2, remap=1, variant=0)),
+    (2, 513, "fp32", dict(bx=64, by=4, remap=1, variant=0)),
+])
+def test_default_launch_shapes(hip, dim, n, prec, want):
+    """lbm_create's default launch (the z-march for 3D FP32 where it
+    applies, narrow blocks for small 2D grids), as include/lbm_capi.h
+    states it."""
+    p = hip.PREC_FP64 if prec == "fp64" else hip.PREC_FP32_SHIFTED
+    s = hip.Solver(dim, n, 1e5, precision=p)
+    got = s.get_launch()
+    s.close()
+    assert {k: got[k] for k in want} == want
