@@ -164,7 +164,9 @@ def run_steady(args, cfg):
     """The reference's own case run to steady state: 2D 129^2, Ra = 1e4,
     Pr = 0.71, from the reference IC to both residuals <= 1e-8 in FP64,
     checking every 2000 steps, then the FP32 shifted path for the same number
-    of steps with its Nu against the FP64 one, then the learned warm start
+    of steps with its Nu against the FP64 one; FP32 through the transient
+    (to a residual of 1e-4, 1e-5 or 2e-6) and then FP64 from the promoted
+    state (lbm_copy_state) to the same rule; and the learned warm start
     against the cold start (both FP64, same rule).  The value is the FP64
     cold wall time from lbm_create to the converged check."""
     import lbmhip
@@ -207,6 +209,27 @@ def run_steady(args, cfg):
                              "bounds it); check intervals replay as HIP graphs"},
         "check": runs,
     }
+    # FP32 through the transient, then FP64 (lbm_copy_state) to the same rule
+    mixed = []
+    for switch in (1e-4, 1e-5, 2e-6):
+        t0 = time.perf_counter()
+        a = lbmhip.Solver(2, cfg["n"], cfg["Ra"], epsU=switch, epsT=switch, itc_max=2000000,
+                          device=0)
+        a.init_rest()
+        st32, itc32, _, _ = a.run()
+        b = lbmhip.Solver(2, cfg["n"], cfg["Ra"], precision=lbmhip.PREC_FP64, epsU=1e-8,
+                          epsT=1e-8, itc_max=2000000, device=0)
+        b.copy_state_from(a)
+        a.close()
+        st, itc, eu, et = b.run()
+        wall = time.perf_counter() - t0
+        nu = b.nusselt()
+        b.close()
+        mixed.append({"fp32_to": switch, "fp32_status": st32, "fp32_itc": itc32, "status": st,
+                      "itc": itc, "errU": eu, "errT": et, "wall_s": round(wall, 4),
+                      "nu_rel_err_vs_fp64": abs(nu[0] / r["nu_hot"] - 1.0)})
+    out["fp32_then_fp64"] = mixed
+
     # the learned warm start on the same case, FP64 to the same rule
     import surrogate
     weights = os.path.join(os.path.dirname(surrogate.WEIGHTS), "warm257.npz")

@@ -168,6 +168,17 @@ int32_t lbm_state_size(lbm_handle h, int64_t *bytes);
 int32_t lbm_save_state(lbm_handle h, void *buf, int64_t bytes);
 int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes);
 
+/* dst takes src's whole solver state (populations, macroscopic fields,
+ * residual snapshots, itc), converted between the storages when their
+ * precisions differ: FP32 shifted f - w_a and g - w_a Tref to FP64 f and g
+ * (each rounded once to FP64), or back (rounded once to FP32); an FP64 dst
+ * of an FP32 src gets the lagged force g beta T rho from the fields.  One
+ * precision: an exact copy.  Both handles need the same grid, physics
+ * and device.  For running FP32 through a transient and FP64 to a residual
+ * below the FP32 floor.  No reference counterpart (the reference has one
+ * precision and no restart). */
+int32_t lbm_copy_state(lbm_handle dst, lbm_handle src);
+
 /* Launch configuration of the step kernel (no reference counterpart: the
  * reference's loops have no launch shape).  One lattice node per thread,
  * bx x by threads per workgroup (bx a multiple of 32, bx*by a multiple of 64

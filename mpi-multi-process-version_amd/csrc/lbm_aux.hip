@@ -135,6 +135,76 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double *__re
     block_reduce_store<NQ>(acc, out);
 }
 
+// One solver state in another's storage (lbm_copy_state): populations as
+// stored by src (SS; shifted: f - w_a and g - w_a Tref, Tref the FP32
+// conduction profile both handles hold) converted to dst's storage (DS),
+// and, for an FP64 destination, the lagged force F = g beta T rho of the
+// FP64 path from the macroscopic fields of the same step (which the FP32
+// path recovers from the node's own populations).
+template <int DIM, typename SS, bool SSHIFT, typename DS, bool DSHIFT>
+__global__ __launch_bounds__(256) void k_convert_state(const SS *__restrict__ src,
+                                                       DS *__restrict__ dst, long plane, long nn,
+                                                       int n, const float *__restrict__ tref,
+                                                       const double *__restrict__ rho,
+                                                       const double *__restrict__ T,
+                                                       double *__restrict__ force, double gbeta) {
+    using FL = Flow<DIM>;
+    using HT = Heat<DIM>;
+    const long node = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nn) return;
+    const int ih = DIM == 2 ? (int)(node % n) : (int)((node / n) % n);
+    const double tl = (double)tref[ih];
+    static_for<0, FL::Q>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        constexpr double wa = FL::w[al];
+        const double f = (double)src[al * plane + node] + (SSHIFT ? wa : 0.0);
+        dst[al * plane + node] = (DS)(DSHIFT ? f - wa : f);
+    });
+    static_for<0, HT::Q>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        constexpr double wa = HT::w[al];
+        const double ws = wa * tl;
+        const double g = (double)src[(FL::Q + al) * plane + node] + (SSHIFT ? ws : 0.0);
+        dst[(FL::Q + al) * plane + node] = (DS)(DSHIFT ? g - ws : g);
+    });
+    if (force) force[node] = gbeta * T[node] * rho[node];
+}
+
+template <int DIM>
+static hipError_t convert_state(const Ctx &dst, const Ctx &src) {
+    dim3 block(256), grid((unsigned)((src.nn + 255) / 256));
+    const void *s = src.pbuf[src.cur];
+    void *d = dst.pbuf[dst.cur];
+    // the FP64 path's force, from the macroscopic fields (FP32 to FP64)
+    double *force = dst.fp64 ? dst.force : nullptr;
+    const double gb = dst.info[LBM_INFO_GBETA];
+    if (src.fp64)
+        hipLaunchKernelGGL((k_convert_state<DIM, double, false, float, true>), grid, block, 0,
+                           dst.stream, (const double *)s, (float *)d, src.plane, src.nn, src.n,
+                           src.d_tref, src.rho, src.T, force, gb);
+    else
+        hipLaunchKernelGGL((k_convert_state<DIM, float, true, double, false>), grid, block, 0,
+                           dst.stream, (const float *)s, (double *)d, src.plane, src.nn, src.n,
+                           src.d_tref, src.rho, src.T, force, gb);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_state(const Ctx &dst, const Ctx &src) {
+    if (src.fp64 == dst.fp64) {
+        // one storage: a plain copy (FP32 (f + w) - w would round deviations
+        // far below w), and the FP64 force as it is
+        const size_t esz = src.fp64 ? sizeof(double) : sizeof(float);
+        hipError_t e = hipMemcpyAsync(dst.pbuf[dst.cur], src.pbuf[src.cur],
+                                      esz * (size_t)src.plane * (size_t)(src.qf + src.qg),
+                                      hipMemcpyDeviceToDevice, dst.stream);
+        if (e == hipSuccess && src.force && dst.force)
+            e = hipMemcpyAsync(dst.force, src.force, sizeof(double) * (size_t)src.nn,
+                               hipMemcpyDeviceToDevice, dst.stream);
+        return e;
+    }
+    return src.dim == 2 ? convert_state<2>(dst, src) : convert_state<3>(dst, src);
+}
+
 hipError_t launch_init_rest(const Ctx &c) {
     dim3 block(256), grid((unsigned)((c.nn + 255) / 256));
     hipLaunchKernelGGL(k_init_rest, grid, block, 0, c.stream, c.rho, c.u, c.v, c.w, c.T,

@@ -716,6 +716,32 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes) {
     return LBM_OK;
 }
 
+int32_t lbm_copy_state(lbm_handle dst, lbm_handle src) {
+    int32_t rc = check_handle(src, true);
+    if (rc) return rc;
+    if ((rc = check_handle(dst, false))) return rc;
+    if (dst == src) return fail(LBM_ERR_ARG, "source and destination are the same handle");
+    const lbm_params &a = src->p, &b = dst->p;
+    if (a.dim != b.dim || a.nx != b.nx || a.Ra != b.Ra || a.Pr != b.Pr || a.Ma != b.Ma ||
+        a.Thot != b.Thot || a.Tcold != b.Tcold || a.stretch != b.stretch ||
+        a.grid_kind != b.grid_kind || a.device != b.device)
+        return fail(LBM_ERR_ARG, "lbm_copy_state needs the same grid, physics and device");
+    HIP_TRY(hipStreamSynchronize(src->stream));
+    HIP_TRY(hipStreamSynchronize(dst->stream));
+    hipError_t e = lbm::launch_convert_state(*dst, *src);
+    if (e != hipSuccess) return hip_fail(e, "state conversion");
+    const size_t fb = sizeof(double) * (size_t)src->nn;
+    double *from[] = {src->rho, src->u, src->v, src->w, src->T, src->up, src->vp, src->wp, src->Tp};
+    double *to[] = {dst->rho, dst->u, dst->v, dst->w, dst->T, dst->up, dst->vp, dst->wp, dst->Tp};
+    for (int k = 0; k < 9; k++)
+        if (from[k] && to[k])
+            HIP_TRY(hipMemcpyAsync(to[k], from[k], fb, hipMemcpyDeviceToDevice, dst->stream));
+    HIP_TRY(hipStreamSynchronize(dst->stream));
+    dst->itc = src->itc;
+    dst->initialized = true;
+    return LBM_OK;
+}
+
 // Launch variant 1 (LDS-staged z-march) stages rows with 16-byte loads, so
 // rows must be 16-byte aligned (n % 4 == 0); the far-wall node then never
 // sits in the first column or row of a tile and its one-sided stencil stays

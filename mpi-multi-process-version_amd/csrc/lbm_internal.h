@@ -92,6 +92,9 @@ hipError_t launch_residual(const Ctx &c);   // results: d_red[0..3]
 // arithmetic); LBM_ERR_DIVERGED if either is not finite.
 int32_t residual_ratios(const double r[4], double *errU, double *errT);
 hipError_t launch_nusselt(const Ctx &c);    // results: d_red[4..5]
+// dst's current population buffer (and, for an FP64 dst, its force) from
+// src's state, converted between the storages; same grid and parameters.
+hipError_t launch_convert_state(const Ctx &dst, const Ctx &src);
 
 // Batched step (lbm_step_f32.hip): one launch advances every cavity listed
 // in the tile table.  tiles[b] = (cavity, tile x, tile y, argument index of

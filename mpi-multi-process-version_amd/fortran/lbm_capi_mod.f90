@@ -119,6 +119,10 @@ module lbm_capi
             type(c_ptr), value :: h, buf
             integer(c_int64_t), value :: bytes
         end function
+        integer(c_int32_t) function lbm_copy_state(dst, src) bind(C, name='lbm_copy_state')
+            import :: c_int32_t, c_ptr
+            type(c_ptr), value :: dst, src
+        end function
         integer(c_int32_t) function lbm_set_launch(h, launch) bind(C, name='lbm_set_launch')
             import :: c_int32_t, c_ptr, lbm_launch
             type(c_ptr), value :: h

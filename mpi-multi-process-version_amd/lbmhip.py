@@ -35,7 +35,7 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_get_grid", "lbm_get_info",
            "lbm_init_rest", "lbm_init_fields", "lbm_step", "lbm_check",
            "lbm_nusselt", "lbm_get_fields", "lbm_run", "lbm_get_itc",
            "lbm_time_steps", "lbm_synchronize", "lbm_state_size",
-           "lbm_save_state", "lbm_load_state", "lbm_set_launch",
+           "lbm_save_state", "lbm_load_state", "lbm_copy_state", "lbm_set_launch",
            "lbm_get_launch", "lbm_autotune", "lbm_bytes_per_update",
            "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_cavity",
            "lbm_batch_init_rest", "lbm_batch_step", "lbm_batch_check",
@@ -100,6 +100,7 @@ def load():
             "lbm_state_size": [H, ctypes.POINTER(I64)],
             "lbm_save_state": [H, ctypes.c_void_p, I64],
             "lbm_load_state": [H, ctypes.c_void_p, I64],
+            "lbm_copy_state": [H, H],
             "lbm_bytes_per_update": [H, DP],
             "lbm_set_launch": [H, ctypes.POINTER(Launch)],
             "lbm_get_launch": [H, ctypes.POINTER(Launch)],
@@ -288,6 +289,11 @@ class Solver:
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
         _check(self._L.lbm_load_state(self.h, buf.ctypes.data_as(ctypes.c_void_p),
                                       buf.size))
+
+    def copy_state_from(self, src):
+        """This solver takes src's state (lbm_copy_state), converted to this
+        solver's precision: e.g. an FP64 solver continuing an FP32 run."""
+        _check(self._L.lbm_copy_state(self.h, src.h))
 
 
 class Batch:

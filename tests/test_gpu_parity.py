@@ -468,3 +468,93 @@ def test_default_launch_shapes(hip, dim, n, prec, want):
     got = s.get_launch()
     s.close()
     assert {k: got[k] for k in want} == want
+
+
+@pytest.mark.parametrize("dim,n,prec", [(2, 33, "fp64"), (2, 65, "fp32"), (3, 32, "fp32"),
+                                        (3, 17, "fp64")])
+def test_copy_state_same_precision_continues_exactly(hip, dim, n, prec):
+    """lbm_copy_state between two handles of one precision (a plain device
+    copy): the copy steps on bit for bit like the original."""
+    p = hip.PREC_FP64 if prec == "fp64" else hip.PREC_FP32_SHIFTED
+    a = hip.Solver(dim, n, 1e5, precision=p)
+    b = hip.Solver(dim, n, 1e5, precision=p)
+    a.init_rest()
+    a.step(150)
+    b.copy_state_from(a)
+    assert b.itc == 150
+    a.step(100)
+    b.step(100)
+    for x, y in zip(a.fields(), b.fields()):
+        assert np.array_equal(x, y)
+    assert a.check() == b.check()
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("dim,n,steps", [(2, 65, 500), (3, 24, 200)])
+def test_copy_state_promotes_fp32_to_fp64(hip, dim, n, steps):
+    """An FP32 run promoted to FP64 half way tracks a pure FP64 run from the
+    same start to within the FP32 bar."""
+    f32 = hip.Solver(dim, n, 1e5)
+    f64 = hip.Solver(dim, n, 1e5, precision=hip.PREC_FP64)
+    ref = hip.Solver(dim, n, 1e5, precision=hip.PREC_FP64)
+    f32.init_rest()
+    ref.init_rest()
+    f32.step(steps // 2)
+    f64.copy_state_from(f32)
+    for x, y in zip(f32.fields(), f64.fields()):
+        assert np.array_equal(x, y)   # the FP64 fields are copied as they are
+    f64.step(steps - steps // 2)
+    ref.step(steps)
+    errs = field_errors(f64.fields(), ref.fields())
+    print(f"\n[promote] {dim}D n={n}: {errs}", flush=True)
+    assert max(errs.values()) <= FP32_TOL
+    for s in (f32, f64, ref):
+        s.close()
+
+
+def test_copy_state_rejects_mismatched_handles(hip):
+    a = hip.Solver(2, 33, 1e5)
+    b = hip.Solver(2, 35, 1e5, precision=hip.PREC_FP64)
+    c = hip.Solver(2, 33, 1e4, precision=hip.PREC_FP64)
+    with pytest.raises(hip.LbmError) as e:
+        b.copy_state_from(a)     # a not initialised
+    assert e.value.code == hip.LBM_ERR_STATE
+    a.init_rest()
+    for dst in (b, c, a):
+        with pytest.raises(hip.LbmError) as e:
+            dst.copy_state_from(a)
+        assert e.value.code == hip.LBM_ERR_ARG
+    for s in (a, b, c):
+        s.close()
+
+
+def test_fp32_then_fp64_reaches_the_fp64_steady_state(hip):
+    """The 129^2, Ra = 1e4 cavity: FP32 to a residual of 2e-6 (above its
+    floor near 1.2e-6), then FP64 from the promoted state to the reference's
+    1e-8.  It ends on the pure FP64 steady state (the oracle's, bit for bit,
+    in test_gpu_steady.py)."""
+    print("\n[fp32 -> fp64] 129^2 Ra 1e4 to 1e-8", flush=True)
+    ref = hip.Solver(2, 129, 1e4, precision=hip.PREC_FP64, epsU=1e-8, epsT=1e-8,
+                     itc_max=2000000)
+    ref.init_rest()
+    st, itc_ref, _, _ = ref.run()
+    assert st == hip.LBM_OK and itc_ref == 536000
+    a = hip.Solver(2, 129, 1e4, epsU=2e-6, epsT=2e-6, itc_max=2000000)
+    a.init_rest()
+    st, itc32, _, _ = a.run()
+    assert st == hip.LBM_OK
+    b = hip.Solver(2, 129, 1e4, precision=hip.PREC_FP64, epsU=1e-8, epsT=1e-8,
+                   itc_max=2000000)
+    b.copy_state_from(a)
+    st, itc, eu, et = b.run()
+    print(f"  FP32 to 2e-6 at {itc32}, FP64 to 1e-8 at {itc} (pure FP64: {itc_ref})", flush=True)
+    assert st == hip.LBM_OK and eu <= 1e-8 and et <= 1e-8
+    errs = field_errors(b.fields(), ref.fields())
+    nb, nr = b.nusselt(), ref.nusselt()
+    print(f"  fields vs the FP64 steady state {errs}, Nu {abs(nb[0] / nr[0] - 1):.2e}",
+          flush=True)
+    assert max(errs.values()) <= FP32_TOL
+    assert abs(nb[0] / nr[0] - 1) <= 1e-6
+    for s in (ref, a, b):
+        s.close()
