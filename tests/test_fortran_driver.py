@@ -206,3 +206,33 @@ def test_driver_3d_fp32_matches_oracle(tmp_path, oracle_mod):
     assert re.fullmatch(r" ( \d\.\d{4}E[-+]\d\d ){2}(( |-)\d\.\d{6}E[-+]\d\d ){6}", raw[0]), raw[0]
     assert data.shape == (17 ** 3, 8)
     o.close()
+
+
+@pytest.mark.gpu
+def test_driver_switches_from_fp32_to_fp64(tmp_path, oracle_mod):
+    """switch_eps: the FP32 run hands its state to FP64 once both residuals
+    are at or below 1e-4, then stops on the reference's 1e-8 rule; the
+    result is the FP64 steady state (the oracle's, run to the same rule)."""
+    print("\n[driver] 2D 33^2 Ra=1e4 FP32 -> FP64 at 1e-4, to 1e-8", flush=True)
+    r = _run(tmp_path, dim=2, nx=33, Ra=1e4, precision=0, switch_eps=1e-4, epsU=1e-8,
+             epsT=1e-8, itc_max=400000, output="binary")
+    print(r.stdout[-800:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "switched to FP64 at itc =" in r.stdout
+    got = _check_lines(r.stdout)
+    assert got[-1][1] <= 1e-8 and got[-1][2] <= 1e-8
+    at = int(re.search(r"switched to FP64 at itc =\s*(\d+)", r.stdout).group(1))
+    row = [g for g in got if g[0] == at][0]
+    assert row[1] <= 1e-4 and row[2] <= 1e-4 and at < got[-1][0]
+    o = oracle_mod.Oracle(2, 33, 1e4)
+    o.init_rest()
+    itc, eu, et = o.run(1e-8, 1e-8, 400000)
+    b = _read_binary(str(tmp_path / f"lbm_fields-{got[-1][0]}.bin"))
+    print(f"  switched at {at}, stopped at {got[-1][0]} (FP64 oracle: {itc})", flush=True)
+    nh, _ = o.nusselt()
+    assert abs(b["nu"][0] / nh - 1) <= 1e-6
+    rho, u, v, w, T = o.fields()
+    vs = np.abs(np.stack([u, v])).max()
+    assert np.abs(b["fields"]["u"] - u.ravel()).max() / vs <= 1e-5
+    assert np.abs(b["fields"]["T"] - T.ravel()).max() <= 1e-5
+    o.close()
