@@ -120,6 +120,73 @@ template <int MT> __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     }
 }
 
+// The forward convolution's product without its im2col matrix: the right
+// operand B(k, n) = X[c][b][yo s + ky - p][xo s + kx - p] (k = (c KS + ky) KS
+// + kx, n = (b Ho + yo) Wo + xo) is gathered from the activations and
+// rounded to BF16 where k_gemm would read it from col: the same operands,
+// MFMAs and order, so the same results.
+struct ConvGeom {
+    const float *X;
+    int B, H, W, stride, pad, Ho, Wo;
+};
+
+template <int MT, int KS>
+__global__ __launch_bounds__(256) void k_gemm_conv(GemmArgs g, ConvGeom cg) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int tn = (g.N + 15) >> 4, tg = (g.M + 16 * MT - 1) / (16 * MT);
+    if (wave >= tn * tg) return;   // whole waves only
+    const int mb = (wave / tn) * 16 * MT, n0 = (wave % tn) << 4;
+    const int kb = blockIdx.z * g.kchunk, ke = min(g.K, kb + g.kchunk);
+    const int r = lane & 15, q = lane >> 4;
+    const int n = n0 + r;
+    const bool nok = n < g.N;
+    const int nn = nok ? n : 0;
+    const int t2 = nn / cg.Wo, xo = nn - t2 * cg.Wo;
+    const int b = t2 / cg.Ho, yo = t2 - b * cg.Ho;
+    const int y0 = yo * cg.stride - cg.pad, x0 = xo * cg.stride - cg.pad;
+    const long bstride = (long)cg.B * cg.H * cg.W;   // one channel
+    const float *Xb = cg.X + (long)b * cg.H * cg.W;
+    const int nt = min(MT, (g.M - mb + 15) >> 4);
+    const bool avec = g.as_k == 1 && (g.as_m & 3) == 0 && ((uintptr_t)g.A & 7) == 0 &&
+                      (ke & 3) == 0;
+    v4f acc[MT];
+#pragma unroll
+    for (int t = 0; t < MT; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        const int k = k0 + 4 * q;
+        v4s bf;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int kk = k + e;
+            const int c = kk / (KS * KS), rr = kk - c * (KS * KS);
+            const int ky = rr / KS, kx = rr - ky * KS;
+            const int y = y0 + ky, x = x0 + kx;
+            float v = 0.0f;
+            if (nok && kk < ke && y >= 0 && y < cg.H && x >= 0 && x < cg.W)
+                v = Xb[c * bstride + y * cg.W + x];
+            bf[e] = (short)to_bf16(v);
+        }
+#pragma unroll
+        for (int t = 0; t < MT; t++) {
+            if (t >= nt) break;
+            const long m = mb + 16 * t + r;
+            const v4s a = fragment(g.A, g.as_m, g.as_k, m, m < g.M, k, ke, avec);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bf, acc[t], 0, 0, 0);
+        }
+    }
+    float *C = g.C + (long)blockIdx.z * g.M * g.ldc;
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+        if (t >= nt) break;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const long mi = mb + 16 * t + 4 * q + e, nj = n0 + r;
+            if (mi < g.M && nj < g.N) C[mi * g.ldc + nj] = acc[t][e];
+        }
+    }
+}
+
 // C[m ldc + n] = sum_z P[z M N + m N + n], z in order
 __global__ void k_reduce_splits(const float *P, float *C, long ldc, int M, int N, int splits) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -409,7 +476,8 @@ uint64_t splitmix(uint64_t &s) {
 constexpr long kPartFloats = 64L * 64 * 128 * 9;
 
 hipError_t gemm(lbm_nn n, int M, int N, int K, const bf16 *A, long as_m, long as_k,
-                const bf16 *B, long bs_k, long bs_n, float *C, long ldc) {
+                const bf16 *B, long bs_k, long bs_n, float *C, long ldc,
+                const nn::ConvGeom *conv = nullptr, int ks = 0) {
     // Waves per launch: one per MT*16 x 16 block of C and k range.  Split
     // the k range (at least 256 long, into the partials buffer) until a
     // launch has about kWaves waves; take four tiles per wave unless one
@@ -436,7 +504,16 @@ hipError_t gemm(lbm_nn n, int M, int N, int K, const bf16 *A, long as_m, long as
                    splits > 1 ? (long)N : ldc, M, N, K, kchunk};
     const long blocks = ((M + 16L * mt - 1) / (16L * mt)) * tn;
     dim3 grid((unsigned)((blocks + 3) / 4), 1, (unsigned)splits);
-    if (mt == 4)
+    if (conv && ks != 1 && ks != 3) return hipErrorInvalidValue;   // kernels for 1x1 and 3x3
+    if (conv && ks == 3 && mt == 4)
+        hipLaunchKernelGGL((nn::k_gemm_conv<4, 3>), grid, dim3(256), 0, n->s, g, *conv);
+    else if (conv && ks == 3)
+        hipLaunchKernelGGL((nn::k_gemm_conv<1, 3>), grid, dim3(256), 0, n->s, g, *conv);
+    else if (conv && mt == 4)
+        hipLaunchKernelGGL((nn::k_gemm_conv<4, 1>), grid, dim3(256), 0, n->s, g, *conv);
+    else if (conv)
+        hipLaunchKernelGGL((nn::k_gemm_conv<1, 1>), grid, dim3(256), 0, n->s, g, *conv);
+    else if (mt == 4)
         hipLaunchKernelGGL(nn::k_gemm<4>, grid, dim3(256), 0, n->s, g);
     else
         hipLaunchKernelGGL(nn::k_gemm<1>, grid, dim3(256), 0, n->s, g);
@@ -461,8 +538,10 @@ void im2col(lbm_nn n, const float *X, int B, int H, int ks, int stride, int pad,
 hipError_t conv_fwd(lbm_nn n, const ConvLayer &l, int B, const float *X, int H, float *Y) {
     const int Ho = H / l.stride, pad = l.ks / 2;
     const long N = (long)B * Ho * Ho, K = (long)l.cin * l.ks * l.ks;
-    im2col(n, X, B, H, l.ks, l.stride, pad, Ho, K);
-    hipError_t e = gemm(n, l.cout, (int)N, (int)K, n->wb + l.woff, K, 1, n->col, N, 1, Y, N);
+    // the product gathers its right operand from X (no im2col matrix)
+    const nn::ConvGeom cg{X, B, H, H, l.stride, pad, Ho, Ho};
+    hipError_t e = gemm(n, l.cout, (int)N, (int)K, n->wb + l.woff, K, 1, nullptr, N, 1, Y, N,
+                        &cg, l.ks);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(nn::k_bias_act, dim3(nn::grid_for((long)l.cout * N)), dim3(256), 0, n->s,
                        Y, (const float *)(n->w + l.boff), l.cout, N, l.relu);
