@@ -558,3 +558,24 @@ def test_fp32_then_fp64_reaches_the_fp64_steady_state(hip):
     assert abs(nb[0] / nr[0] - 1) <= 1e-6
     for s in (ref, a, b):
         s.close()
+
+
+@pytest.mark.parametrize("dim,n", [(2, 65), (3, 24)])
+def test_copy_state_demotes_fp64_to_fp32(hip, dim, n):
+    """An FP64 state demoted to FP32 storage (each population rounded once)
+    steps on within the FP32 bar of the FP64 run it came from."""
+    f64 = hip.Solver(dim, n, 1e5, precision=hip.PREC_FP64)
+    f32 = hip.Solver(dim, n, 1e5)
+    f64.init_rest()
+    f64.step(200)
+    f32.copy_state_from(f64)
+    assert f32.itc == 200
+    f64.step(100)
+    f32.step(100)
+    errs = field_errors(f32.fields(), f64.fields())
+    print(f"\n[demote] {dim}D n={n}: {errs}", flush=True)
+    assert max(errs.values()) <= FP32_TOL
+    nu32, nu64 = f32.nusselt(), f64.nusselt()
+    assert abs(nu32[0] / nu64[0] - 1) <= NU_TOL
+    f64.close()
+    f32.close()
