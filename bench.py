@@ -201,9 +201,9 @@ def run_steady(args, cfg):
         "config": {"workload": cfg["label"], "nodes": nodes, "grid": "erf A=3.2",
                    "Ra": cfg["Ra"], "Pr": 0.71, "check_every": 2000,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(224 * nodes * r["itc"] / r["wall_s"] / 1e9, 2),
+        "roofline": {"bound": "hbm", "achieved": round(240 * nodes * r["itc"] / r["wall_s"] / 1e9, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(224 * nodes * r["itc"] / r["wall_s"] / 1e9 / HBM_PEAK_GBS, 5),
+                     "frac": round(240 * nodes * r["itc"] / r["wall_s"] / 1e9 / HBM_PEAK_GBS, 5),
                      "traffic": None,
                      "note": "16 641 nodes: each step is one short kernel (its latency, not HBM, "
                              "bounds it); check intervals replay as HIP graphs"},

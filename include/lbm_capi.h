@@ -215,7 +215,8 @@ int32_t lbm_get_launch(lbm_handle h, lbm_launch *launch);
 int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms);
 
 /* Bytes moved per lattice update by one step kernel, as counted for the
- * roofline: every population read once and written once. */
+ * roofline: every population read once and written once, plus, in FP64
+ * storage, the per-node force read and written (16 B). */
 int32_t lbm_bytes_per_update(lbm_handle h, double *bytes);
 
 /* ------------------------------------------------------------------------
@@ -242,10 +243,13 @@ int32_t lbm_batch_destroy(lbm_batch b);
  * through the batch. */
 lbm_handle lbm_batch_cavity(lbm_batch b, int32_t index);
 
-/* lbm_init_rest on every cavity; all cavities become active. */
+/* lbm_init_rest on every cavity. */
 int32_t lbm_batch_init_rest(lbm_batch b);
 
-/* nsteps time steps of every active cavity (asynchronous). */
+/* nsteps time steps of every active cavity (asynchronous).  Active means
+ * initialised (lbm_batch_init_rest, or lbm_init_rest / lbm_init_fields /
+ * lbm_load_state on lbm_batch_cavity) and not retired by lbm_batch_run
+ * since its last initialisation.  LBM_ERR_STATE if no cavity is active. */
 int32_t lbm_batch_step(lbm_batch b, int64_t nsteps);
 
 /* lbm_check on every cavity (errU, errT: count values each). */
@@ -255,16 +259,18 @@ int32_t lbm_batch_check(lbm_batch b, double *errU, double *errT);
 int32_t lbm_batch_nusselt(lbm_batch b, double *nu_hot, double *nu_cold);
 
 /* Runs every cavity with the stopping rule of lbm_run, each with its own
- * epsU, epsT and itc_max: the active cavities step together in windows of
- * check_every; a cavity whose residuals both reach its eps, whose itc
- * reaches its itc_max, or whose residual is non-finite leaves the batch.
+ * epsU, epsT and itc_max: every initialised cavity below its itc_max
+ * steps with the others in windows of check_every; a cavity whose
+ * residuals both reach its eps, whose itc reaches its itc_max, or whose
+ * residual is non-finite leaves the batch.  At the end every cavity is
+ * retired: lbm_batch_step skips it until it is initialised again.
  * itc, errU, errT, status (LBM_OK, LBM_ERR_NOT_CONVERGED or
  * LBM_ERR_DIVERGED): count values each, any may be NULL.  Returns LBM_OK
  * when the run completed, whatever the per-cavity statuses. */
 int32_t lbm_batch_run(lbm_batch b, int64_t *itc, double *errU, double *errT, int32_t *status);
 
-/* Number of active cavities and their total node count (the MLUPS
- * numerator of one step). */
+/* Number of active cavities (those lbm_batch_step advances) and their
+ * total node count (the MLUPS numerator of one step). */
 int32_t lbm_batch_active(lbm_batch b, int32_t *cavities, int64_t *nodes);
 
 /* lbm_time_steps for the batch: nsteps batched steps between two HIP events

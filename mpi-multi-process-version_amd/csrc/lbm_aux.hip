@@ -212,18 +212,21 @@ hipError_t launch_init_rest(const Ctx &c) {
     return hipGetLastError();
 }
 
-hipError_t launch_residual(const Ctx &c) {
+// The residual's two kernels with their partial sums in `part` (4 per
+// block) and the four sums in `red`.
+static hipError_t residual_into(const Ctx &c, double *part, double *red) {
     dim3 block(kRedThreads), grid(c.red_blocks);
     if (c.dim == 2)
         hipLaunchKernelGGL(k_resid_part<2>, grid, block, 0, c.stream, c.u, c.v, c.w, c.T,
-                           c.up, c.vp, c.wp, c.Tp, c.nn, c.d_part);
+                           c.up, c.vp, c.wp, c.Tp, c.nn, part);
     else
         hipLaunchKernelGGL(k_resid_part<3>, grid, block, 0, c.stream, c.u, c.v, c.w, c.T,
-                           c.up, c.vp, c.wp, c.Tp, c.nn, c.d_part);
-    hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), block, 0, c.stream, c.d_part,
-                       c.red_blocks, c.d_red);
+                           c.up, c.vp, c.wp, c.Tp, c.nn, part);
+    hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), block, 0, c.stream, part, c.red_blocks, red);
     return hipGetLastError();
 }
+
+hipError_t launch_residual(const Ctx &c) { return residual_into(c, c.d_part, c.d_red); }
 
 hipError_t launch_nusselt(const Ctx &c) {
     dim3 block(kRedThreads), grid(c.red_blocks);

@@ -20,7 +20,12 @@ using lbm::set_error;
 
 struct lbm_batch_ctx {
     std::vector<lbm_handle> cav;
+    // Cavities a step advances: every initialised cavity, except those
+    // lbm_batch_run retired (converged, at itc_max or diverged) and that
+    // have not been initialised again since (retired_at holds the cavity's
+    // initialisation count at retirement, -1 if not retired).
     std::vector<char> active;
+    std::vector<int64_t> retired_at;
     std::vector<int> base_cur;   // cur of each cavity when the tile table was built
     int device = 0;
     int check_every = 2000;
@@ -37,8 +42,18 @@ struct lbm_batch_ctx {
     // (73.8 against 76.9) and 64 of 257^2 (137.8 against 145.7; 128 x 4:
     // 132.4)
     int bx = 64, by = 8;
-    int variant = 0;   // 0: one node per thread; 1: y-march strips
+    int variant = 0;   // 0: one node per thread; 1: y-march strips; 2: persistent
     int kc = 0;        // variant 1: rows per chunk
+    // variant 2 (one persistent launch per lbm_batch_step call): per-cavity
+    // (first table entry, gx, gy), the MALL group offsets, the ticket
+    // counter and abort flag, the per-tile step counters, and the resident
+    // workgroup count for the current block shape
+    int4 *d_cav = nullptr;
+    int *d_groups = nullptr, *d_ctl = nullptr, *d_done = nullptr;
+    int groups_cap = 0, done_cap = 0;
+    int *h_ctl = nullptr;   // pinned: the abort flag, read at synchronising calls
+    int nwg = 0, nwg_bx = 0, nwg_by = 0;
+    uint64_t spin_ticks = 0;   // 2 s of the device's wall clock
     double *h_red = nullptr;   // pinned host memory: 4 residual sums per cavity
     // variant 0: tile-table offsets of the MALL groups (whole cavities in
     // table order), groups[g] .. groups[g + 1]; one group when all fit
@@ -47,6 +62,16 @@ struct lbm_batch_ctx {
 };
 
 namespace {
+
+// Variant 2, the persistent launch (k_step_batch_persist), is switched off.
+// Its first two GPU runs, inside lbm_batch_autotune, stopped making
+// progress: first with acquire loads in the dependency polls (each poll
+// then invalidates the XCD's L2), then again with relaxed polls, one
+// acquire fence per wait and a 2 s wall-clock bound on every wait, which
+// should have turned any wait that never ends into an error.  The cause is
+// not established, so lbm_batch_set_launch refuses the variant and the
+// autotuner skips it.
+constexpr bool kPersistentEnabled = false;
 
 // bytes a 2D FP32 step moves per node: 14 populations read and written
 constexpr double kStepBytesPerNode = 14 * 4.0 * 2.0;
@@ -77,6 +102,10 @@ void release(lbm_batch b) {
     }
     if (b->d_args) (void)hipFree(b->d_args);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
+    void *persist[] = {b->d_cav, b->d_groups, b->d_ctl, b->d_done};
+    for (void *q : persist)
+        if (q) (void)hipFree(q);
+    if (b->h_ctl) (void)hipHostFree(b->h_ctl);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->h_red) (void)hipHostFree(b->h_red);
@@ -90,11 +119,106 @@ int32_t check_batch(lbm_batch b) {
     return LBM_OK;
 }
 
+bool steppable(const lbm_batch b, size_t c) {
+    const lbm_ctx &h = *b->cav[c];
+    return h.initialized && b->retired_at[c] != h.inits;
+}
+
+// The active set outside lbm_batch_run: every steppable cavity.
+void refresh_active(lbm_batch b) {
+    for (size_t c = 0; c < b->cav.size(); c++) {
+        const char a = steppable(b, c) ? 1 : 0;
+        if (a != b->active[c]) {
+            b->active[c] = a;
+            b->dirty = true;
+        }
+    }
+}
+
+int32_t need_active(lbm_batch b) {
+    refresh_active(b);
+    for (char a : b->active)
+        if (a) return LBM_OK;
+    return set_error(LBM_ERR_STATE, "no cavity to step: initialise one (lbm_batch_init_rest, or "
+                                    "lbm_init_rest/lbm_init_fields/lbm_load_state on "
+                                    "lbm_batch_cavity) or re-initialise a retired one");
+}
+
 bool table_stale(const lbm_batch b) {
     if (b->dirty) return true;
     for (size_t c = 0; c < b->cav.size(); c++)
         if (b->active[c] && b->cav[c]->cur != (b->base_cur[c] ^ b->parity)) return true;
     return false;
+}
+
+// Variant 2's device tables for the tile table just built: each cavity's
+// first entry and tile grid, the MALL group offsets, one step counter per
+// tile, and (once) the ticket counter and the abort flag.
+int32_t persist_tables(lbm_batch b, const std::vector<int4> &tiles) {
+    const size_t nc = b->cav.size();
+    std::vector<int4> cav(nc, make_int4(0, 0, 0, 0));
+    for (size_t e = 0; e < tiles.size(); e++) {
+        const int4 t = tiles[e];
+        if (t.y == 0 && t.z == 0) {
+            const int n = b->cav[t.x]->n;
+            cav[t.x] = make_int4((int)e, (n + b->bx - 1) / b->bx, (n + b->by - 1) / b->by, 0);
+        }
+    }
+    if (!b->d_cav) BATCH_TRY(hipMalloc(&b->d_cav, sizeof(int4) * nc));
+    BATCH_TRY(hipMemcpy(b->d_cav, cav.data(), sizeof(int4) * nc, hipMemcpyHostToDevice));
+    if ((int)b->groups.size() > b->groups_cap) {
+        if (b->d_groups) BATCH_TRY(hipFree(b->d_groups));
+        b->d_groups = nullptr;
+        b->groups_cap = 0;
+        BATCH_TRY(hipMalloc(&b->d_groups, sizeof(int) * b->groups.size()));
+        b->groups_cap = (int)b->groups.size();
+    }
+    BATCH_TRY(hipMemcpy(b->d_groups, b->groups.data(), sizeof(int) * b->groups.size(),
+                        hipMemcpyHostToDevice));
+    if (b->ntiles > b->done_cap) {
+        if (b->d_done) BATCH_TRY(hipFree(b->d_done));
+        b->d_done = nullptr;
+        b->done_cap = 0;
+        BATCH_TRY(hipMalloc(&b->d_done, sizeof(int) * (size_t)b->ntiles));
+        b->done_cap = b->ntiles;
+    }
+    if (!b->d_ctl) {
+        BATCH_TRY(hipMalloc(&b->d_ctl, 2 * sizeof(int)));
+        BATCH_TRY(hipMemset(b->d_ctl, 0, 2 * sizeof(int)));
+    }
+    return LBM_OK;
+}
+
+// Resident workgroups of the persistent kernel for the current block
+// shape, over all compute units.
+int32_t persist_workgroups(lbm_batch b, bool nt) {
+    if (b->nwg > 0 && b->nwg_bx == b->bx && b->nwg_by == b->by) return LBM_OK;
+    int per_cu = 0, cus = 0;
+    BATCH_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, lbm::step_batch_persist_kernel(nt), b->bx * b->by, 0));
+    BATCH_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    int khz = 0;
+    BATCH_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+    // a wait lasts a few tile updates (microseconds); 2 s only ends a
+    // launch that cannot make progress
+    b->spin_ticks = (uint64_t)std::max(khz, 1000) * 2000ULL;
+    b->nwg = std::max(1, per_cu) * std::max(1, cus);
+    b->nwg_bx = b->bx;
+    b->nwg_by = b->by;
+    return LBM_OK;
+}
+
+// LBM_ERR_HIP if a persistent launch of this batch gave up on a dependency
+// wait (the fields are then invalid).  Synchronises the batch's stream.
+int32_t persist_status(lbm_batch b) {
+    if (!b->d_ctl) return LBM_OK;
+    BATCH_TRY(hipMemcpyAsync(b->h_ctl, b->d_ctl + 1, sizeof(int), hipMemcpyDeviceToHost,
+                             b->stream));
+    BATCH_TRY(hipStreamSynchronize(b->stream));
+    if (*b->h_ctl)
+        return set_error(LBM_ERR_HIP, "persistent batch launch: a tile's dependency wait timed "
+                                      "out and the launch stopped; the fields are invalid");
+    return LBM_OK;
 }
 
 // Lists the tiles of every active cavity, largest cavity first.  Variant 1
@@ -163,6 +287,10 @@ int32_t rebuild(lbm_batch b) {
     b->ntiles = (int)tiles.size();
     b->groups.push_back(b->ntiles);
     b->group_nodes.push_back(gnodes);
+    if (b->variant == 2) {
+        int32_t rc = persist_tables(b, tiles);
+        if (rc) return rc;
+    }
     b->parity = 0;
     b->dirty = false;
     return LBM_OK;
@@ -181,6 +309,32 @@ int32_t batch_steps(lbm_batch b, int64_t nsteps) {
     for (size_t c = 0; c < b->cav.size(); c++)
         if (b->active[c]) nodes += (double)b->cav[c]->nn;
     const int ng = (int)b->groups.size() - 1;
+    if (b->variant == 2) {
+        // one persistent launch for all nsteps (several only if the ticket
+        // count would pass 2^31)
+        const bool nt = ng <= 1 && lbm::stream_stores(nodes * kStepBytesPerNode);
+        int32_t rc = persist_workgroups(b, nt);
+        if (rc) return rc;
+        const int64_t per = std::max<int64_t>(1, (INT32_MAX - 2LL * b->nwg) / std::max(1, b->ntiles));
+        for (int64_t s0 = 0; s0 < nsteps; s0 += per) {
+            const int64_t d = std::min(per, nsteps - s0);
+            lbm::PersistPlan pl{b->d_tiles, b->d_cav, b->d_groups, ng, b->ntiles, (int)d,
+                                ng > 1 ? (int)kGroupDepth : (int)d, b->parity,
+                                b->by >= 2 ? 1 : 2, b->d_ctl, b->d_done, b->spin_ticks};
+            const int64_t items = d * (int64_t)b->ntiles;
+            const int nwg = (int)std::min<int64_t>(b->nwg, items);
+            hipError_t e = lbm::launch_step_batch_persist_f32(b->d_args, pl, b->bx, b->by, nwg,
+                                                              nt, b->stream);
+            if (e != hipSuccess) return hip_err(e, "persistent batch launch");
+            b->parity ^= (int)(d & 1);
+        }
+        for (size_t c = 0; c < b->cav.size(); c++) {
+            if (!b->active[c]) continue;
+            b->cav[c]->cur ^= (int)(nsteps & 1);
+            b->cav[c]->itc += nsteps;
+        }
+        return LBM_OK;
+    }
     auto launch = [&](int g0, int g1, int parity, bool macro, bool nt) {
         const int t0 = b->groups[g0], cnt = b->groups[g1] - t0;
         return b->variant == 1
@@ -263,10 +417,12 @@ int32_t lbm_batch_create(const lbm_params *params, int32_t count, lbm_batch *out
         b->cav[c]->stream = b->stream;
     }
     b->active.assign(count, 0);
+    b->retired_at.assign(count, -1);
     b->base_cur.assign(count, 0);
     e = hipHostMalloc((void **)&b->h_red, sizeof(double) * 4 * (size_t)count);
+    if (e != hipSuccess) b->h_red = nullptr;
+    else if ((e = hipHostMalloc((void **)&b->h_ctl, sizeof(int))) != hipSuccess) b->h_ctl = nullptr;
     if (e != hipSuccess) {
-        b->h_red = nullptr;
         release(b);
         return hip_err(e, "batch residual buffer");
     }
@@ -309,11 +465,9 @@ lbm_handle lbm_batch_cavity(lbm_batch b, int32_t index) {
 int32_t lbm_batch_init_rest(lbm_batch b) {
     int32_t rc = check_batch(b);
     if (rc) return rc;
-    for (size_t c = 0; c < b->cav.size(); c++) {
+    for (size_t c = 0; c < b->cav.size(); c++)
         if ((rc = lbm_init_rest(b->cav[c]))) return rc;
-        b->active[c] = 1;
-    }
-    b->dirty = true;
+    refresh_active(b);
     return LBM_OK;
 }
 
@@ -321,6 +475,8 @@ int32_t lbm_batch_step(lbm_batch b, int64_t nsteps) {
     int32_t rc = check_batch(b);
     if (rc) return rc;
     if (nsteps < 0) return set_error(LBM_ERR_ARG, "nsteps < 0");
+    if (nsteps == 0) return LBM_OK;
+    if ((rc = need_active(b))) return rc;
     return batch_steps(b, nsteps);
 }
 
@@ -334,7 +490,7 @@ int32_t lbm_batch_check(lbm_batch b, double *errU, double *errT) {
         if (errT) errT[c] = et;
         if (rc && rc != LBM_ERR_DIVERGED) return rc;
     }
-    return LBM_OK;
+    return persist_status(b);
 }
 
 int32_t lbm_batch_nusselt(lbm_batch b, double *nu_hot, double *nu_cold) {
@@ -347,7 +503,7 @@ int32_t lbm_batch_nusselt(lbm_batch b, double *nu_hot, double *nu_cold) {
         if (nu_cold) nu_cold[c] = k;
         if (rc && rc != LBM_ERR_DIVERGED) return rc;
     }
-    return LBM_OK;
+    return persist_status(b);
 }
 
 int32_t lbm_batch_run(lbm_batch b, int64_t *itc, double *errU, double *errT, int32_t *status) {
@@ -360,6 +516,7 @@ int32_t lbm_batch_run(lbm_batch b, int64_t *itc, double *errU, double *errT, int
     for (size_t c = 0; c < nc; c++) {
         if (!b->cav[c]->initialized) return set_error(LBM_ERR_STATE, "cavity not initialised");
         b->active[c] = b->cav[c]->itc < b->cav[c]->p.itc_max;
+        b->retired_at[c] = -1;
     }
     b->dirty = true;
     for (;;) {
@@ -384,7 +541,10 @@ int32_t lbm_batch_run(lbm_batch b, int64_t *itc, double *errU, double *errT, int
             if (e != hipSuccess) return hip_err(e, "batch residual");
             due = true;
         }
-        if (due) BATCH_TRY(hipStreamSynchronize(b->stream));
+        if (due) {
+            BATCH_TRY(hipStreamSynchronize(b->stream));
+            if ((rc = persist_status(b))) return rc;
+        }
         bool changed = false;
         for (size_t c = 0; c < nc; c++) {
             if (!b->active[c]) continue;
@@ -411,6 +571,10 @@ int32_t lbm_batch_run(lbm_batch b, int64_t *itc, double *errU, double *errT, int
         if (changed) b->dirty = true;
     }
     BATCH_TRY(hipStreamSynchronize(b->stream));
+    if ((rc = persist_status(b))) return rc;
+    // every cavity has left the run: later steps skip it until it is
+    // initialised again
+    for (size_t c = 0; c < nc; c++) b->retired_at[c] = b->cav[c]->inits;
     for (size_t c = 0; c < nc; c++) {
         if (itc) itc[c] = b->cav[c]->itc;
         if (errU) errU[c] = eu[c];
@@ -422,6 +586,7 @@ int32_t lbm_batch_run(lbm_batch b, int64_t *itc, double *errU, double *errT, int
 
 int32_t lbm_batch_active(lbm_batch b, int32_t *cavities, int64_t *nodes) {
     if (!b) return set_error(LBM_ERR_ARG, "null batch");
+    refresh_active(b);
     int32_t k = 0;
     int64_t m = 0;
     for (size_t c = 0; c < b->cav.size(); c++)
@@ -437,11 +602,14 @@ int32_t lbm_batch_active(lbm_batch b, int32_t *cavities, int64_t *nodes) {
 int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms) {
     int32_t rc = check_batch(b);
     if (rc) return rc;
+    if (nsteps < 1 || !ms) return set_error(LBM_ERR_ARG, "nsteps >= 1 and ms required");
+    if ((rc = need_active(b))) return rc;
     if (table_stale(b) && (rc = rebuild(b))) return rc;   // the timed region only launches
     BATCH_TRY(hipEventRecord(b->ev0, b->stream));
     if ((rc = batch_steps(b, nsteps))) return rc;
     BATCH_TRY(hipEventRecord(b->ev1, b->stream));
     BATCH_TRY(hipEventSynchronize(b->ev1));
+    if ((rc = persist_status(b))) return rc;
     float t = 0.0f;
     BATCH_TRY(hipEventElapsedTime(&t, b->ev0, b->ev1));
     if (ms) *ms = t;
@@ -451,10 +619,14 @@ int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms) {
 int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *l) {
     if (!b || !l) return set_error(LBM_ERR_ARG, "null argument");
     if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
-        l->remap != 0 || l->occupancy != 0 || l->variant < 0 || l->variant > 1 ||
+        l->remap != 0 || l->occupancy != 0 || l->variant < 0 || l->variant > 2 ||
         (l->variant == 1 && (l->bx != 64 || l->by != 1)))
         return set_error(LBM_ERR_ARG, "invalid batched launch configuration (variant 1: bx = 64, "
                                       "by = 1)");
+    if (l->variant == 2 && !kPersistentEnabled)
+        return set_error(LBM_ERR_ARG, "variant 2 (the persistent launch) is switched off: its "
+                                      "first GPU runs stopped making progress and the cause is "
+                                      "not established");
     b->bx = l->bx;
     b->by = l->by;
     b->variant = l->variant;
@@ -472,6 +644,7 @@ int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms
     int32_t rc = check_batch(b);
     if (rc) return rc;
     if (steps_per_trial < 1) return set_error(LBM_ERR_ARG, "steps_per_trial < 1");
+    if ((rc = need_active(b))) return rc;
     const size_t nc = b->cav.size();
     // host snapshot of every initialised cavity
     std::vector<std::vector<char>> saved(nc);
@@ -482,14 +655,17 @@ int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms
         saved[c].resize((size_t)bytes);
         if ((rc = lbm_save_state(b->cav[c], saved[c].data(), bytes))) return rc;
     }
-    // (bx, by, variant); variant 1 is the y-march
+    // (bx, by, variant); variant 1 is the y-march, variant 2 the
+    // persistent launch
     static const int cand[][3] = {{64, 1, 0},  {64, 2, 0},  {64, 4, 0},  {64, 8, 0},
                                   {128, 1, 0}, {128, 2, 0}, {128, 4, 0}, {256, 1, 0},
-                                  {256, 2, 0}, {32, 8, 0},  {64, 1, 1}};
+                                  {256, 2, 0}, {32, 8, 0},  {64, 1, 1},  {64, 4, 2},
+                                  {64, 8, 2},  {128, 4, 2}, {32, 8, 2}};
     const int bx0 = b->bx, by0 = b->by, v0 = b->variant;
     int best_bx = bx0, best_by = by0, best_v = v0;
     double best = 1e300;
     for (const auto &cd : cand) {
+        if (cd[2] == 2 && !kPersistentEnabled) continue;
         b->bx = cd[0];
         b->by = cd[1];
         b->variant = cd[2];
@@ -511,8 +687,12 @@ int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms
     BATCH_TRY(hipStreamSynchronize(b->stream));
     for (size_t c = 0; c < nc; c++) {
         if (saved[c].empty()) continue;
+        // restoring a state is not a new initialisation: a retired cavity
+        // stays retired
+        const bool retired = b->retired_at[c] == b->cav[c]->inits;
         int32_t r2 = lbm_load_state(b->cav[c], saved[c].data(), (int64_t)saved[c].size());
         if (r2 && !rc) rc = r2;
+        if (retired) b->retired_at[c] = b->cav[c]->inits;
     }
     if (rc) return rc;
     if (best_ms) *best_ms = best;

@@ -206,6 +206,7 @@ int32_t run_prologue(Ctx &c) {
     if (c.wp) HIP_TRY(hipMemsetAsync(c.wp, 0, bytes, c.stream));
     c.itc = 0;
     c.initialized = true;
+    c.inits++;
     return LBM_OK;
 }
 
@@ -713,6 +714,7 @@ int32_t lbm_load_state(lbm_handle h, const void *buf, int64_t bytes) {
     }
     h->itc = hd.itc;
     h->initialized = true;
+    h->inits++;
     return LBM_OK;
 }
 
@@ -739,6 +741,7 @@ int32_t lbm_copy_state(lbm_handle dst, lbm_handle src) {
     HIP_TRY(hipStreamSynchronize(dst->stream));
     dst->itc = src->itc;
     dst->initialized = true;
+    dst->inits++;
     return LBM_OK;
 }
 
@@ -863,7 +866,10 @@ int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms) {
 int32_t lbm_bytes_per_update(lbm_handle h, double *bytes) {
     if (!h || !bytes) return fail(LBM_ERR_ARG, "null argument");
     const double esz = h->fp64 ? 8.0 : 4.0;
-    *bytes = 2.0 * esz * (h->qf + h->qg);
+    // every population read once and written once; the FP64 path also
+    // reads and writes its per-node force (the FP32 path recovers it from
+    // the node's own populations)
+    *bytes = 2.0 * esz * (h->qf + h->qg) + (h->fp64 ? 2.0 * sizeof(double) : 0.0);
     return LBM_OK;
 }
 

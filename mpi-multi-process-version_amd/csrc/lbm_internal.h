@@ -52,6 +52,9 @@ struct Ctx {
     int red_blocks = 0;
     int64_t itc = 0;
     bool initialized = false;
+    int64_t inits = 0;          // initialisations so far (init_rest/init_fields/
+                                // load_state/copy_state), so a batch can tell a
+                                // re-initialised cavity from one it retired
     LaunchShape shape{};
     StepArgs<float> a32{};
     StepArgs<double> a64{};
@@ -108,6 +111,33 @@ hipError_t launch_step_batch_ym_f32(bool macro, const StepArgs<float> *args, con
                                     int ntiles, int parity, int kc, bool nt, hipStream_t s);
 // Rows per chunk of the y-march for a total of strip_rows (strips x rows).
 int ym_rows_per_chunk(double strip_rows);
+
+// The batch's variant 2: one persistent launch of `steps` time steps over
+// the variant-0 tile table (tiles[e] = (cavity, tile x, tile y, argument
+// index at parity 0), each cavity's tiles contiguous, y-major), cut into
+// MALL groups groups[0..ngroups] and chunks of `depth` steps.  cav[c] =
+// (first table entry, gx, gy) of cavity c.  ctl[0] is the ticket counter,
+// ctl[1] the sticky abort flag (a dependency wait timed out); done[e] the
+// last step tile e completed in this launch.  parity is the buffer parity
+// of the launch's first step.  A wait longer than spin_ticks aborts.
+struct PersistPlan {
+    const int4 *tiles;
+    const int4 *cav;
+    const int *groups;
+    int ngroups, ntiles;
+    int steps, depth;
+    int parity;
+    int ry;
+    int *ctl;
+    int *done;
+    uint64_t spin_ticks;   // wait bound, in ticks of the device's wall clock
+};
+// Resets the ticket counter and the step counters, then launches nwg
+// persistent workgroups of bx x by threads.
+hipError_t launch_step_batch_persist_f32(const StepArgs<float> *args, const PersistPlan &pl,
+                                         int bx, int by, int nwg, bool nt, hipStream_t s);
+// The persistent kernel, for occupancy queries.
+const void *step_batch_persist_kernel(bool nt);
 
 constexpr int kRedThreads = 256;
 constexpr int kRedMaxBlocks = 1024;

@@ -173,6 +173,152 @@ hipError_t launch_step_batch_f32(bool macro, const StepArgs<float> *args, const 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------
+// The batch's variant 2: one persistent launch advances every cavity of
+// the batch through `steps` time steps.  Work items are (step, tile) pairs
+// handed out by one atomic ticket counter, in the order
+//   chunk of `depth` steps, MALL group, step, tile of the group,
+// so every item a tile's update depends on (the tiles around it, one step
+// earlier) holds a smaller ticket.  A workgroup takes a ticket, waits until
+// the tiles within the stencil's reach have finished the previous step
+// (per-tile step counters), updates its tile with the single-cavity node
+// code and publishes its own counter.  Waits only ever target smaller
+// tickets, which running workgroups hold, so the launch completes whatever
+// the number of resident workgroups.  Every wait is bounded in time
+// (pl.spin_ticks of the constant-rate wall clock): past it the workgroup
+// raises the abort flag, every workgroup then stops taking tickets, and the
+// host reports the error.
+//
+// Memory ordering across the XCDs (each with its own L2): the counters are
+// polled with relaxed device-scope loads, which read the coherent value
+// without cache maintenance, and one device-scope acquire fence follows the
+// wait (on gfx950 it invalidates the L1 and L2 lines that may be stale).
+// Polling with acquire loads would invalidate the XCD's L2 on every poll
+// and starve the workgroups that do the work.  Before the release, every
+// wave waits for its own stores (vmcnt(0)), so that the lead wave's
+// device-scope release (an L2 write-back) covers the whole tile.
+//
+// Reach: a node's stencil spans its neighbours, and the one-sided wall
+// stencils reach two nodes inward, so with tiles of at least 2 rows (and
+// bx >= 32) the tiles within one tile in x and y cover it; 1-row tiles need
+// two rows of tiles (ry = 2).  The same set covers the write-after-read
+// hazard: a tile's step-s output overwrites the buffer that its neighbours
+// read at step s - 1.
+
+// s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left at their
+// maxima (gfx9 encoding: vmcnt [3:0] and [15:14], expcnt [6:4], lgkmcnt
+// [11:8]).
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+__device__ __forceinline__ bool persist_decode(const PersistPlan &pl, long k, int &e, int &s) {
+    const long T = pl.ntiles, DT = (long)pl.depth * T;
+    const int nfull = pl.steps / pl.depth;
+    long r;
+    int chunk, d;
+    if (k < (long)nfull * DT) {
+        chunk = (int)(k / DT);
+        r = k - (long)chunk * DT;
+        d = pl.depth;
+    } else {
+        chunk = nfull;
+        r = k - (long)nfull * DT;
+        d = pl.steps - nfull * pl.depth;
+        if (r >= (long)d * T) return false;
+    }
+    int g = 0;
+    while (g + 1 < pl.ngroups && r >= (long)d * pl.groups[g + 1]) g++;
+    const long g0 = pl.groups[g], ng = pl.groups[g + 1] - g0;
+    const long rr = r - (long)d * g0;
+    s = chunk * pl.depth + (int)(rr / ng) + 1;
+    e = (int)(g0 + rr % ng);
+    return true;
+}
+
+template <bool NT>
+__global__ __launch_bounds__(1024) void k_step_batch_persist(
+    const StepArgs<float> *__restrict__ args, PersistPlan pl) {
+    __shared__ int item[2];
+    const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+    for (;;) {
+        if (lead) {
+            int e = -1, s = 0;
+            if (!__hip_atomic_load(&pl.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                const long k = __hip_atomic_fetch_add(&pl.ctl[0], 1, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+                if (!persist_decode(pl, k, e, s)) e = -1;
+            }
+            if (e >= 0 && s > 1) {
+                const int4 t = pl.tiles[e];
+                const int4 ci = pl.cav[t.x];   // first entry, gx, gy
+                const uint64_t t0 = wall_clock64();
+                bool ok = true;
+                for (int dy = -pl.ry; dy <= pl.ry && ok; dy++) {
+                    const int ty = t.z + dy;
+                    if (ty < 0 || ty >= ci.z) continue;
+                    for (int dx = -1; dx <= 1 && ok; dx++) {
+                        const int tx = t.y + dx;
+                        if (tx < 0 || tx >= ci.y) continue;
+                        const int *flag = pl.done + ci.x + ty * ci.y + tx;
+                        while (__hip_atomic_load(flag, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) < s - 1) {
+                            if (wall_clock64() - t0 > pl.spin_ticks ||
+                                __hip_atomic_load(&pl.ctl[1], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)) {
+                                __hip_atomic_store(&pl.ctl[1], 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                                ok = false;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(2);
+                        }
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (!ok) e = -1;
+            }
+            item[0] = e;
+            item[1] = s;
+        }
+        __syncthreads();
+        const int e = item[0], s = item[1];
+        if (e < 0) break;
+        const int4 t = pl.tiles[e];
+        const StepArgs<float> a = args[t.w ^ ((pl.parity + s - 1) & 1)];
+        const int i = t.y * (int)blockDim.x + (int)threadIdx.x;
+        const int j = t.z * (int)blockDim.y + (int)threadIdx.y;
+        if (i < a.n && j < a.n) {
+            if (s == pl.steps) step_node<2, float, float, true, true, NT>(a, i, j, 0);
+            else step_node<2, float, float, true, false, NT>(a, i, j, 0);
+        }
+        // every wave's stores of the tile have reached the L2 (vmcnt(0))
+        // before the lead wave's release writes the L2 back
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        __syncthreads();
+        if (lead) __hip_atomic_store(pl.done + e, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k_persist_reset(int *__restrict__ ctl, int *__restrict__ done, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) ctl[0] = 0;   // the ticket counter; the abort flag ctl[1] is sticky
+    for (int k = t; k < n; k += gridDim.x * blockDim.x) done[k] = 0;
+}
+
+const void *step_batch_persist_kernel(bool nt) {
+    return nt ? (const void *)k_step_batch_persist<true> : (const void *)k_step_batch_persist<false>;
+}
+
+hipError_t launch_step_batch_persist_f32(const StepArgs<float> *args, const PersistPlan &pl,
+                                         int bx, int by, int nwg, bool nt, hipStream_t s) {
+    if (pl.ntiles <= 0 || pl.steps <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_persist_reset, dim3(std::min(64, (pl.ntiles + 255) / 256)), dim3(256),
+                       0, s, pl.ctl, pl.done, pl.ntiles);
+    dim3 block(bx, by, 1), grid(nwg);
+    if (nt) hipLaunchKernelGGL(k_step_batch_persist<true>, grid, block, 0, s, args, pl);
+    else hipLaunchKernelGGL(k_step_batch_persist<false>, grid, block, 0, s, args, pl);
+    return hipGetLastError();
+}
+
 hipError_t launch_prologue_f32(int dim, const StepArgs<float> &a,
                                const StepArgs<double> &ad, hipStream_t s) {
     long nn = dim == 2 ? (long)a.n * a.n : (long)a.n * a.n * a.n;

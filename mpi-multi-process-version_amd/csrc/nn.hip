@@ -806,8 +806,11 @@ int32_t lbm_nn_train_step(lbm_nn n, int32_t batch, const float *x, const float *
     if ((rc = forward(n, batch, x)) || (rc = backward(n, batch, target, loss))) return rc;
     n->t++;
     const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.0f - (float)std::pow(0.9, (double)n->t);
-    const float c2 = 1.0f - (float)std::pow(0.999, (double)n->t);
+    // bias corrections 1 - beta^t of the decay rates the kernel applies,
+    // formed in FP64 and rounded once: 1.0f - (float)beta^t would lose
+    // five digits to cancellation (1 - 0.999^2 = 0.001999)
+    const float c1 = (float)(1.0 - std::pow((double)b1, (double)n->t));
+    const float c2 = (float)(1.0 - std::pow((double)b2, (double)n->t));
     hipLaunchKernelGGL(nn::k_adam, dim3(nn::grid_for(n->np)), dim3(256), 0, n->s, n->w,
                        (const float *)n->g, n->m, n->v, n->np, lr, b1, b2, 1e-8f, c1, c2);
     NN_TRY(hipGetLastError());

@@ -48,6 +48,7 @@ struct orc {
     long nn;
     long itc;
     double Ra, Pr, Ma, Thot, Tcold, A;
+    int grid_kind;   /* 0: erf map (the reference's), 1: tanh map */
     double p[ORC_P_COUNT];
     int c[QMAX][3];
     double wf[QMAX], wg[7];
@@ -109,13 +110,18 @@ static void moment_row_3d(const int *c, double *row) {
 /* ------------------------------------------------------------------- grid */
 /* mesh (2M:74-134, 3M:122-189): erf-stretched unit interval, smallest
  * spacing dx0 = X1 - X0, length_LB = 1/dx0, dt = dx0*length_LB, nodes
- * x_i = (X_i - dx0/2)*length_LB for i = 1..n. */
+ * x_i = (X_i - dx0/2)*length_LB for i = 1..n.  grid_kind 1 puts tanh in
+ * place of erf in the map of 2M:85-91 (an extension: the reference has the
+ * erf map only); everything after the map is the same. */
 static void make_grid(orc *s) {
     int n = s->n;
     double A = s->A;
     double *X = (double *)malloc(sizeof(double) * (n + 2));
-    for (int i = 0; i <= n + 1; i++)
-        X[i] = 0.5 * (erf(A * ((double)i / (double)(n + 1) - 0.5)) / erf(0.5 * A) + 1.0);
+    for (int i = 0; i <= n + 1; i++) {
+        double t = A * ((double)i / (double)(n + 1) - 0.5);
+        X[i] = s->grid_kind == 1 ? 0.5 * (tanh(t) / tanh(0.5 * A) + 1.0)
+                                 : 0.5 * (erf(t) / erf(0.5 * A) + 1.0);
+    }
     double dx0 = X[1] - X[0];
     double L = 1.0 / dx0;
     s->p[ORC_P_DX0] = dx0;
@@ -170,8 +176,14 @@ static void make_params(orc *s) {
 
 orc *orc_create(int dim, int n, double Ra, double Pr, double Ma,
                 double Thot, double Tcold, double A) {
-    if ((dim != 2 && dim != 3) || n < 3) return NULL;
+    return orc_create_grid(dim, n, Ra, Pr, Ma, Thot, Tcold, A, 0);
+}
+
+orc *orc_create_grid(int dim, int n, double Ra, double Pr, double Ma,
+                     double Thot, double Tcold, double A, int grid_kind) {
+    if ((dim != 2 && dim != 3) || n < 3 || grid_kind < 0 || grid_kind > 1) return NULL;
     orc *s = (orc *)calloc(1, sizeof(orc));
+    s->grid_kind = grid_kind;
     s->dim = dim;
     s->n = n;
     s->qf = dim == 2 ? 9 : 19;

@@ -44,6 +44,11 @@ enum {
 
 orc *orc_create(int dim, int n, double Ra, double Pr, double Ma,
                 double Thot, double Tcold, double A);
+/* The same with a choice of grid map: 0 = erf (2M:85-91, what orc_create
+ * uses), 1 = tanh in place of erf (an extension with no reference
+ * counterpart). */
+orc *orc_create_grid(int dim, int n, double Ra, double Pr, double Ma,
+                     double Thot, double Tcold, double A, int grid_kind);
 void orc_destroy(orc *s);
 void orc_set_threads(int nthreads);   /* OpenMP threads for the sweeps (>=1) */
 int  orc_get_threads(void);

@@ -35,6 +35,9 @@ def lib():
         DP = ctypes.POINTER(ctypes.c_double)
         L.orc_create.restype = P
         L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, D]
+        L.orc_create_grid.restype = P
+        L.orc_create_grid.argtypes = [ctypes.c_int, ctypes.c_int, D, D, D, D, D, D,
+                                      ctypes.c_int]
         L.orc_destroy.argtypes = [P]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         for name in ["orc_dim", "orc_n", "orc_qf", "orc_qg"]:
@@ -77,9 +80,9 @@ class Oracle:
     (n, n[, n]) for fields with index order [k, j, i] (i fastest)."""
 
     def __init__(self, dim, n, Ra, Pr=0.71, Ma=0.1, Thot=1.0, Tcold=0.0,
-                 A=3.2):
+                 A=3.2, grid_kind=0):
         self._l = lib()
-        self.h = self._l.orc_create(dim, n, Ra, Pr, Ma, Thot, Tcold, A)
+        self.h = self._l.orc_create_grid(dim, n, Ra, Pr, Ma, Thot, Tcold, A, grid_kind)
         if not self.h:
             raise ValueError("orc_create rejected the arguments")
         self.dim, self.n = dim, n

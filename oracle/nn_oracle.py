@@ -141,3 +141,17 @@ def loss_grad(p, x, t, quant=True):
     bwd(0, mask(da1, A["a1"]), A["x"], need_dx=False)
     g = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in grads])
     return loss, g
+
+
+def adam_step(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma and Ba 2015, Algorithm 1) at step count t >= 1:
+    m <- b1 m + (1-b1) g, v <- b2 v + (1-b2) g^2, then
+    p <- p - lr (m / (1-b1^t)) / (sqrt(v / (1-b2^t)) + eps).  FP64; returns
+    the new (p, m, v).  This is the update lbm_nn_train_step documents
+    (beta1 0.9, beta2 0.999, eps 1e-8, the state cleared by set_params)."""
+    g = np.asarray(g, dtype=np.float64)
+    m = b1 * np.asarray(m, dtype=np.float64) + (1.0 - b1) * g
+    v = b2 * np.asarray(v, dtype=np.float64) + (1.0 - b2) * g * g
+    mhat = m / (1.0 - b1 ** t)
+    vhat = v / (1.0 - b2 ** t)
+    return np.asarray(p, dtype=np.float64) - lr * mhat / (np.sqrt(vhat) + eps), m, v

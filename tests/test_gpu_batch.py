@@ -120,6 +120,54 @@ def test_batch_run_stops_each_cavity_on_its_own_rule(hip):
     b.close()
 
 
+def test_batch_steps_cavities_initialised_one_by_one(hip):
+    """Cavities set up through their own handles (lbm_init_fields on
+    lbm_batch_cavity, no lbm_batch_init_rest) are active: a batch step
+    advances every one of them like a single-cavity solver.  After
+    lbm_batch_run every cavity is retired, lbm_batch_step refuses with
+    LBM_ERR_STATE, and re-initialising one cavity makes it, and only it,
+    active again."""
+    print("\n[batch] init_fields on every cavity, then batch steps", flush=True)
+    cav = CAVITIES[:4]
+    b = hip.Batch(cav)
+    b.set_launch(64, 4)
+    with pytest.raises(hip.LbmError) as e:
+        b.step(1)
+    assert e.value.code == hip.LBM_ERR_STATE
+    rng = np.random.default_rng(11)
+    init = []
+    for i, c in enumerate(cav):
+        n = c["n"]
+        u = 1e-3 * rng.standard_normal((n, n))
+        v = 1e-3 * rng.standard_normal((n, n))
+        T = 0.5 + 0.1 * rng.standard_normal((n, n))
+        init.append((u, v, T))
+        b[i].init_fields(u, v, None, T)
+    assert b.active() == (len(cav), sum(c["n"] ** 2 for c in cav))
+    b.step(40)
+    for i, c in enumerate(cav):
+        assert b[i].itc == 40
+        s = _single(hip, c)
+        s.set_launch(64, 4, remap=0)
+        u, v, T = init[i]
+        s.init_fields(u, v, None, T)
+        s.step(40)
+        errs = [_rel(x, y) for x, y in zip(b[i].fields(), s.fields()) if np.abs(y).max() > 0]
+        assert max(errs) <= 1e-6, (i, errs)
+        s.close()
+    b.run()
+    assert b.active()[0] == 0
+    with pytest.raises(hip.LbmError) as e:
+        b.step(1)
+    assert e.value.code == hip.LBM_ERR_STATE
+    b[2].init_rest()
+    assert b.active() == (1, cav[2]["n"] ** 2)
+    itc0 = b[1].itc
+    b.step(10)
+    assert b[2].itc == 10 and b[1].itc == itc0
+    b.close()
+
+
 def test_batch_warm_start_of_one_cavity(hip):
     print("\n[batch] init_fields on one cavity", flush=True)
     b = hip.Batch(CAVITIES[:3])
@@ -198,3 +246,75 @@ def test_batch_mall_groups_keep_every_cavity_exact(hip):
         s.close()
     for x in [a] + parts:
         x.close()
+
+
+@pytest.mark.skip(reason="variant 2 (the persistent launch) is switched off: its first two GPU runs, in lbm_batch_autotune, stopped making progress and the cause is not established")
+@pytest.mark.parametrize("bx,by", [(64, 8), (64, 1), (32, 2)])
+def test_batch_persistent_launch_matches_per_step_launches(hip, bx, by):
+    """Variant 2 advances the whole batch through every lbm_batch_step call
+    in one persistent launch (a work queue of (step, tile) items with
+    per-tile step counters).  Each node's update is the same code as the
+    per-step launches of variant 0 with the same block, so every cavity must
+    agree with variant 0 to the bit, over an odd and an even step count and
+    with by = 1 (tiles of one row, whose wall stencils reach two tile rows)."""
+    print(f"\n[batch persistent] {bx}x{by} vs per-step launches", flush=True)
+    cav = CAVITIES + [dict(n=129, Ra=1e6, Pr=0.71, stretch=2.0)]
+    a, p = hip.Batch(cav), hip.Batch(cav)
+    a.set_launch(bx, by, variant=0)
+    p.set_launch(bx, by, variant=2)
+    assert p.get_launch()["variant"] == 2
+    for x in (a, p):
+        x.init_rest()
+        x.step(301)
+        x.step(6)
+    for i in range(len(cav)):
+        assert a[i].itc == p[i].itc == 307
+        for x, y in zip(a[i].fields(), p[i].fields()):
+            assert np.array_equal(x, y), (i, cav[i])
+    a.close()
+    p.close()
+
+
+@pytest.mark.skip(reason="variant 2 (the persistent launch) is switched off: its first two GPU runs, in lbm_batch_autotune, stopped making progress and the cause is not established")
+def test_batch_persistent_launch_with_mall_groups(hip):
+    """A batch above half the MALL runs its persistent launch group by group
+    in chunks of 64 steps (ticket order: chunk, group, step, tile).  Every
+    cavity must match the grouped per-step launches to the bit, over full
+    and partial chunks."""
+    cav = [dict(n=513 if i % 3 else 257, Ra=10 ** (3 + 3 * i / 13), Pr=0.7 + 0.4 * i,
+                stretch=1.0 + 0.1 * i) for i in range(14)]
+    print(f"\n[batch persistent, MALL groups] {len(cav)} cavities", flush=True)
+    a, p = hip.Batch(cav), hip.Batch(cav)
+    a.set_launch(128, 4, variant=0)
+    p.set_launch(128, 4, variant=2)
+    for x in (a, p):
+        x.init_rest()
+        x.step(70)
+        x.step(7)
+    for i in range(len(cav)):
+        assert a[i].itc == p[i].itc == 77
+        for x, y in zip(a[i].fields(), p[i].fields()):
+            assert np.array_equal(x, y), i
+    a.close()
+    p.close()
+
+
+@pytest.mark.skip(reason="variant 2 (the persistent launch) is switched off: its first two GPU runs, in lbm_batch_autotune, stopped making progress and the cause is not established")
+def test_batch_persistent_run_stops_like_single_cavities(hip):
+    """lbm_batch_run on variant 2: one persistent launch per check window,
+    each cavity stopped by its own rule, the same iteration counts,
+    statuses and residuals as variant 0."""
+    print("\n[batch persistent] run with per-cavity eps and itc_max", flush=True)
+    cav = [dict(n=17, Ra=1e3, epsU=1e-2, epsT=1e-2, itc_max=40000),
+           dict(n=24, Ra=1e4, epsU=1e-9, epsT=1e-9, itc_max=6000),
+           dict(n=20, Ra=1e4, stretch=1.5, epsU=1e-3, epsT=1e-3, itc_max=40000)]
+    res = []
+    for variant in (0, 2):
+        b = hip.Batch(cav, check_every=1000)
+        b.set_launch(64, 4, variant=variant)
+        b.init_rest()
+        res.append(b.run())
+        b.close()
+    for x, y in zip(res[0], res[1]):
+        assert np.array_equal(x, y)
+    assert res[1][3].tolist() == [hip.LBM_OK, hip.LBM_ERR_NOT_CONVERGED, hip.LBM_OK]

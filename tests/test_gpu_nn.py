@@ -94,3 +94,44 @@ def test_training_reduces_the_loss(nnmod):
     net.set_params(p0)
     assert abs(net.loss_grad(x, t)[0] - first) <= 1e-6 * first
     net.close()
+
+
+def test_adam_step_matches_oracle(nnmod):
+    """lbm_nn_train_step's Adam update (k_adam) against the oracle's textbook
+    step, fed the HIP network's own gradient (lbm_nn_loss_grad at the same
+    parameters: the forward and backward passes are deterministic), over
+    three steps with the oracle carrying m, v and t.  Then set_params clears
+    the state: the next step from the starting point repeats the first one
+    to the bit.  FP32 update arithmetic: 1e-6 of lr per parameter (a wrong
+    bias correction or eps placement is off by at least 10% of lr), plus
+    one FP32 unit of the stored parameter."""
+    print("\n[nn adam] G=16 B=2, 3 steps", flush=True)
+    G, B, lr = 16, 2, 1e-3
+    net = nnmod.Net(grid=G, batch_max=B, seed=5)
+    x, t = _data(G, B, 6)
+    p0 = net.get_params()
+    p = p0.astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    firsts = None
+    for step in range(1, 4):
+        pin = net.get_params()
+        _, g = net.loss_grad(x, t)
+        assert np.array_equal(net.get_params(), pin)
+        net.train_step(x, t, lr)
+        got = net.get_params()
+        want, m, v = O.adam_step(pin, g, m, v, step, lr)
+        # one FP32 unit of the stored parameter plus the update's rounding
+        tol = 2.0 ** -23 * np.abs(want) + 1e-6 * lr
+        err = np.abs(got - want)
+        moved = np.abs(got - pin).max()
+        print(f"  step {step}: max |p - oracle| / tol = {(err / tol).max():.3g} "
+              f"(largest move {moved:.3g})", flush=True)
+        assert np.all(err <= tol)
+        assert moved > 0.5 * lr
+        if step == 1:
+            firsts = got
+    net.set_params(p0)
+    net.train_step(x, t, lr)
+    assert np.array_equal(net.get_params(), firsts)
+    net.close()

@@ -71,6 +71,31 @@ def test_fp64_rest_start_matches_oracle(oracle_mod, hip, dim, n, Ra):
     assert s.itc == o.itc == 100
 
 
+@pytest.mark.parametrize("dim,n,Ra,precision", [(2, 33, 1e5, 1), (3, 13, 1e5, 1),
+                                                (2, 40, 1e5, 0), (3, 16, 1e5, 0)])
+def test_tanh_grid_matches_oracle(oracle_mod, hip, dim, n, Ra, precision):
+    """The tanh grid map (LBM_GRID_TANH, an extension): the grid, weights
+    and trajectory against the oracle's tanh grid, FP64 to 1e-13 and FP32
+    (the 3D case on the z-march) to 1e-5, with the Nusselt numbers."""
+    print(f"\n[tanh grid] dim={dim} n={n} precision={precision}", flush=True)
+    o = oracle_mod.Oracle(dim, n, Ra, A=2.0, grid_kind=1)
+    s = hip.Solver(dim, n, Ra, stretch=2.0, grid_kind=hip.GRID_TANH, precision=precision)
+    assert np.array_equal(s.grid()[0], o.coords())
+    o.init_rest()
+    s.init_rest()
+    o.step(300)
+    s.step(300)
+    errs = field_errors(s.fields(), o.fields())
+    print(f"  errs={errs}", flush=True)
+    tol = FP64_TOL if precision == hip.PREC_FP64 else FP32_TOL
+    for k, e in errs.items():
+        assert e <= tol, (k, e)
+    for a, b in zip(s.nusselt(), o.nusselt()):
+        assert abs(a - b) <= (1e-12 if precision == hip.PREC_FP64 else NU_TOL) * abs(b)
+    s.close()
+    o.close()
+
+
 @pytest.mark.parametrize("dim,n,Ra,seed", [(2, 29, 1e5, 1), (3, 14, 1e5, 2)])
 def test_fp64_warm_start_matches_oracle(oracle_mod, hip, dim, n, Ra, seed):
     print(f"\n[fp64 warm] dim={dim} n={n} Ra={Ra}", flush=True)

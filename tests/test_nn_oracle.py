@@ -112,3 +112,30 @@ def test_header_declares_the_binding_list(hip):
     for n in names:
         assert hasattr(lib, n), n
     assert O.param_count() == 169107
+
+
+def test_adam_closed_forms():
+    """The oracle's Adam step against what the update rule fixes.
+    A gradient held constant for t steps gives bias-corrected moments equal
+    to g and g^2 exactly, so every step moves p by lr g / (|g| + eps): the
+    bias corrections are pinned (without them the first step is 10x (m) or
+    sqrt(1000)x (v) off).  A gradient of 1e-9 moves p by lr / 11 (eps added
+    outside the square root, not inside: that would give lr 1e-9 /
+    sqrt(1e-18 + 1e-8)).  A zero gradient does not move p."""
+    lr = 1e-2
+    g = np.array([3.0, -0.5, 1e-9, 0.0, -2e-3])
+    p = np.array([1.0, -1.0, 0.5, 0.25, 0.0])
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    want_step = lr * g / (np.abs(g) + 1e-8)
+    for t in range(1, 6):
+        p1, m, v = O.adam_step(p, g, m, v, t, lr)
+        np.testing.assert_allclose(p - p1, want_step, rtol=1e-12, atol=1e-18)
+        p = p1
+    step = 0.5 - O.adam_step(np.array([0.5]), np.array([1e-9]), np.zeros(1), np.zeros(1),
+                             1, lr)[0][0]
+    assert abs(step - lr / 11) <= 1e-12 * lr
+    # the moments themselves: after one step from zero, (1-b1) g and (1-b2) g^2
+    _, m1, v1 = O.adam_step(p, g, np.zeros_like(g), np.zeros_like(g), 1, lr)
+    np.testing.assert_allclose(m1, 0.1 * g, rtol=1e-15)
+    np.testing.assert_allclose(v1, 0.001 * g * g, rtol=1e-12)

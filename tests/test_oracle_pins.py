@@ -81,6 +81,41 @@ def test_grid_closed_form(oracle_mod, n):
     assert np.abs(x + x[::-1] - H).max() < 1e-9 * H   # symmetric stretching
 
 
+@pytest.mark.parametrize("n", [5, 33, 129, 256, 257, 1025])
+def test_tanh_grid_closed_form(oracle_mod, n):
+    """The tanh map (grid_kind 1, an extension with no reference
+    counterpart): tanh in place of erf in the map of 2M:85-91, then the
+    reference's dx0, length_LB, dt and node shift.  Pinned by the closed
+    form, dt = 1, the walls half a unit from the end nodes, the mirror
+    symmetry, and the spacing ratio that only the tanh map has: a spacing
+    is the map's derivative at its midpoint to O(h^2), so centre/wall
+    spacings go as sech^2(t_c)/sech^2(t_w) (the erf map gives
+    exp(t_w^2 - t_c^2)), t_k = A (k/(n+1) - 1/2)."""
+    A = 3.2
+    o = oracle_mod.Oracle(2, n, 1e5, A=A, grid_kind=1)
+    p = o.params()
+    x = o.coords()
+    X = [0.5 * (math.tanh(A * (i / (n + 1) - 0.5)) / math.tanh(0.5 * A) + 1.0)
+         for i in range(n + 2)]
+    dx0 = X[1] - X[0]
+    assert p["dx0"] == dx0 and p["L_LB"] == 1.0 / dx0
+    assert abs(p["dt"] - 1.0) <= 2.0 ** -52
+    ref = np.array([(X[i] - dx0 / 2.0) * (1.0 / dx0) for i in range(1, n + 1)])
+    assert np.array_equal(x, ref)
+    H = p["H"]
+    assert abs(x[0] - 0.5) < 1e-12 and abs(x[-1] + 0.5 - H) < 1e-9 * H
+    assert np.all(np.diff(x) > 0)
+    assert np.abs(x + x[::-1] - H).max() < 1e-9 * H
+    if n >= 257:
+        d = np.diff(x)
+        m = len(d) // 2
+        tw, tc = A * (1.5 / (n + 1) - 0.5), A * ((m + 1.5) / (n + 1) - 0.5)
+        want = (math.cosh(tw) / math.cosh(tc)) ** 2
+        assert abs(d[m] / d[0] / want - 1) < 1e-3, (d[m] / d[0], want)
+        de = np.diff(oracle_mod.Oracle(2, n, 1e5, A=A).coords())
+        assert abs(de[m] / de[0] / math.exp(tw * tw - tc * tc) - 1) < 1e-3
+
+
 def test_stencils_are_one_sided_at_walls(oracle_mod):
     n = 9
     st = oracle_mod.Oracle(2, n, 1e5).stencil()
