@@ -46,13 +46,9 @@ WORKLOADS = {
 
 
 def batch_cavities(count, seed):
-    """The batched sweep's cavities: edge from {65, 129, 257, 513}, erf
-    stretch factor A ~ U[1, 2.5], log10 Ra ~ U[3, 6], Pr ~ U[0.7, 7]."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    return [dict(n=int(rng.choice([65, 129, 257, 513])), stretch=float(rng.uniform(1.0, 2.5)),
-                 Ra=float(10.0 ** rng.uniform(3.0, 6.0)), Pr=float(rng.uniform(0.7, 7.0)))
-            for _ in range(count)]
+    """The batched sweep's cavities (lbmhip.sweep_cavities)."""
+    import lbmhip
+    return lbmhip.sweep_cavities(count, seed)
 
 
 def traffic_from_profiles(workload, precision, launch):

@@ -394,3 +394,18 @@ class Batch:
         ms = ctypes.c_double()
         _check(self._L.lbm_batch_autotune(self.h, int(steps_per_trial), ctypes.byref(ms)))
         return ms.value
+
+
+# The batched sweep's seed (configuration C4: 64 cavities).
+SWEEP_SEED = 20261203
+
+
+def sweep_cavities(count=64, seed=SWEEP_SEED):
+    """The batched sweep's cavities: edge from {65, 129, 257, 513}, erf
+    stretch factor A ~ U[1, 2.5], log10 Ra ~ U[3, 6], Pr ~ U[0.7, 7], drawn
+    in that order per cavity from numpy's PCG64 with the given seed."""
+    rng = np.random.default_rng(seed)
+    return [dict(n=int(rng.choice([65, 129, 257, 513])), stretch=float(rng.uniform(1.0, 2.5)),
+                 Ra=float(10.0 ** rng.uniform(3.0, 6.0)), Pr=float(rng.uniform(0.7, 7.0)))
+            for _ in range(count)]
+

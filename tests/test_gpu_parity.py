@@ -80,7 +80,7 @@ def test_tanh_grid_matches_oracle(oracle_mod, hip, dim, n, Ra, precision):
     print(f"\n[tanh grid] dim={dim} n={n} precision={precision}", flush=True)
     o = oracle_mod.Oracle(dim, n, Ra, A=2.0, grid_kind=1)
     s = hip.Solver(dim, n, Ra, stretch=2.0, grid_kind=hip.GRID_TANH, precision=precision)
-    assert np.array_equal(s.grid()[0], o.coords())
+    assert np.array_equal(s.grid(), o.coords())
     o.init_rest()
     s.init_rest()
     o.step(300)
@@ -197,23 +197,43 @@ def test_call_order_and_arguments(hip):
     assert e.value.code == hip.LBM_ERR_ARG
 
 
+def _rel_l2(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
 def test_large_2d_fp32_vs_fp64_gpu(hip):
-    """1025^2 at Ra = 1e6: the FP32 path after 2000 steps against the FP64
-    GPU path (itself checked against the oracle at small sizes above)."""
-    print("\n[2D 1025 fp32 vs fp64] 2000 steps", flush=True)
+    """1025^2 at Ra = 1e6 (configuration C2) over the reference's 20 000 steps
+    (2dim_mono.f90:31): the FP32 path against the FP64 GPU path (itself
+    checked against the oracle at small sizes above), at 2000 steps and at
+    20 000.  Bars: relative L2 error <= 1e-5 for the velocity and for T, and
+    both walls' Nu within 1e-4.  The max-norm error is bounded at 1e-5 up to
+    2000 steps and at 2e-5 over the whole run: through this start-up
+    transient the rounding of the FP32-stored populations alone accumulates
+    to 1.4-2.0e-5 in the largest pointwise velocity error (a build that
+    rounds the FP64 path's stored populations to the FP32 shifted format,
+    with FP64 arithmetic otherwise, gives 2.0e-5), so a 1e-5 max-norm bar at
+    20 000 steps needs FP64 storage (LBM_PREC_FP64)."""
+    print("\n[2D 1025 fp32 vs fp64] 2000 and 20000 steps", flush=True)
     a = hip.Solver(2, 1025, 1e6, precision=hip.PREC_FP32_SHIFTED)
     b = hip.Solver(2, 1025, 1e6, precision=hip.PREC_FP64)
     a.init_rest()
     b.init_rest()
-    a.step(2000)
-    b.step(2000)
-    errs = field_errors(a.fields(), b.fields())
-    print(f"  errs={errs}", flush=True)
-    for k in ("u", "v", "T"):
-        assert errs[k] <= FP32_TOL, (k, errs[k])
-    na, nb = a.nusselt(), b.nusselt()
-    print(f"  Nu fp32={na} fp64={nb}", flush=True)
-    assert abs(na[0] - nb[0]) <= NU_TOL * abs(nb[0])
+    for steps, max_bar in ((2000, FP32_TOL), (20000, 2 * FP32_TOL)):
+        a.step(steps - a.itc)
+        b.step(steps - b.itc)
+        fa, fb = a.fields(), b.fields()
+        errs = field_errors(fa, fb)
+        l2u = float(np.sqrt((np.sum((fa[1] - fb[1]) ** 2) + np.sum((fa[2] - fb[2]) ** 2)) /
+                            (np.sum(fb[1] ** 2) + np.sum(fb[2] ** 2))))
+        l2t = _rel_l2(fa[4], fb[4])
+        na, nb = a.nusselt(), b.nusselt()
+        nu_err = max(abs(x - y) / abs(y) for x, y in zip(na, nb))
+        print(f"  itc={steps}: max-norm {errs}; relative L2 velocity {l2u:.3g} T {l2t:.3g}; "
+              f"Nu fp32={na} fp64={nb} (rel {nu_err:.3g})", flush=True)
+        assert l2u <= FP32_TOL and l2t <= FP32_TOL, (steps, l2u, l2t)
+        for k in ("u", "v", "T"):
+            assert errs[k] <= max_bar, (steps, k, errs[k])
+        assert nu_err <= NU_TOL, (steps, na, nb)
 
 
 def test_large_3d_fp32_vs_fp64_gpu(hip):
