@@ -129,6 +129,17 @@ int32_t lbm_step(lbm_handle h, int64_t nsteps);
  * reported as 0 (the reference would compute 0/0). */
 int32_t lbm_check(lbm_handle h, double *errU, double *errT);
 
+/* lbm_check with nothing allocated by the library for the call: no
+ * reference counterpart; for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  The same kernels and results as lbm_check, bit for
+ * bit, with the reduction's partial sums (at most 1024 blocks x 4 FP64
+ * values, the most lbm_create ever sets up) in one static device array of
+ * 32.8 KB per GPU, held until the process ends, instead of the handle's
+ * buffers.  Calls that overlap on one GPU (handles on different streams
+ * or threads) share that array and race: serialise them. */
+int32_t lbm_check_noalloc(lbm_handle h, double *errU, double *errT);
+
 /* Mean Nusselt numbers of the hot and cold walls, FP64 (no reference
  * counterpart; the reference only writes fields, 2dim_mono.f90:818-843):
  * Nu = H dT/dn / (Thot - Tcold) from the one-sided quadratic through the

@@ -3,6 +3,8 @@
 // deterministic: a fixed grid accumulates per-thread partial sums, reduces
 // them per block with an LDS tree, and one block adds the block results in
 // a fixed order.  Compiled with -ffp-contract=off.
+#include <mutex>
+
 #include "lbm_internal.h"
 
 namespace lbm {
@@ -227,6 +229,49 @@ static hipError_t residual_into(const Ctx &c, double *part, double *red) {
 }
 
 hipError_t launch_residual(const Ctx &c) { return residual_into(c, c.d_part, c.d_red); }
+
+// Scratch memory of lbm_check_noalloc: the residual's partial sums (4 per
+// block, at most kRedMaxBlocks blocks) followed by the 4 sums, one copy per
+// GPU for the life of the process (32.8 KB).  Host code uses the device
+// address from hipGetSymbolAddress (scratch_address), never the array's
+// name, which on the host is only a stand-in.
+constexpr int kScratchDoubles = 4 * kRedMaxBlocks + 4;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on the current device, looked up once
+// per device (with that device current) and kept.
+static hipError_t scratch_address(double **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_residual_scratch(const Ctx &c, double **sums) {
+    if (4 * c.red_blocks + 4 > kScratchDoubles) return hipErrorInvalidValue;
+    double *base = nullptr;
+    hipError_t e = scratch_address(&base);
+    if (e != hipSuccess) return e;
+    // every partial the final kernel reads is written by the first kernel
+    // of the same call, so nothing left by an earlier call needs clearing
+    *sums = base + 4 * kRedMaxBlocks;
+    return residual_into(c, base, *sums);
+}
 
 hipError_t launch_nusselt(const Ctx &c) {
     dim3 block(kRedThreads), grid(c.red_blocks);

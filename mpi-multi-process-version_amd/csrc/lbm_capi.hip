@@ -549,6 +549,21 @@ int32_t lbm_check(lbm_handle h, double *errU, double *errT) {
     return lbm::residual_ratios(r, errU, errT);
 }
 
+int32_t lbm_check_noalloc(lbm_handle h, double *errU, double *errT) {
+    int32_t rc = check_handle(h, true);
+    if (rc) return rc;
+    if (4 * (long)h->red_blocks + 4 > 4 * (long)lbm::kRedMaxBlocks + 4)
+        return fail(LBM_ERR_ARG, "lbm_check_noalloc: more reduction blocks than its scratch "
+                                 "array holds");
+    double *sums = nullptr;
+    hipError_t e = lbm::launch_residual_scratch(*h, &sums);
+    if (e != hipSuccess) return hip_fail(e, "residual launch (static scratch)");
+    double r[4];
+    HIP_TRY(hipMemcpyAsync(r, sums, sizeof r, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return lbm::residual_ratios(r, errU, errT);
+}
+
 int32_t lbm_nusselt(lbm_handle h, double *nu_hot, double *nu_cold) {
     int32_t rc = check_handle(h, true);
     if (rc) return rc;

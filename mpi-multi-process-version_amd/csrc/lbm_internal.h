@@ -91,6 +91,11 @@ hipError_t launch_prologue_f64(int dim, const StepArgs<double> &a,
 // Auxiliary kernels (lbm_aux.hip), all FP64.
 hipError_t launch_init_rest(const Ctx &c);
 hipError_t launch_residual(const Ctx &c);   // results: d_red[0..3]
+// The same kernels with their partial sums and results in the library's
+// static device scratch array of the current device instead of d_part and
+// d_red; *sums receives the device address of the four sums.  Allocates
+// nothing.
+hipError_t launch_residual_scratch(const Ctx &c, double **sums);
 // errU and errT from the four sums of launch_residual (lbm_check's
 // arithmetic); LBM_ERR_DIVERGED if either is not finite.
 int32_t residual_ratios(const double r[4], double *errU, double *errT);

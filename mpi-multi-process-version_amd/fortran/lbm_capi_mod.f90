@@ -74,6 +74,14 @@ module lbm_capi
             type(c_ptr), value :: h
             real(c_double), intent(out) :: errU, errT
         end function
+        ! lbm_check with the reduction's partial sums in the library's static
+        ! device array (no allocation; calls that overlap on one GPU race)
+        integer(c_int32_t) function lbm_check_noalloc(h, errU, errT) &
+                bind(C, name='lbm_check_noalloc')
+            import :: c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: errU, errT
+        end function
         integer(c_int32_t) function lbm_nusselt(h, nu_hot, nu_cold) bind(C, name='lbm_nusselt')
             import :: c_int32_t, c_ptr, c_double
             type(c_ptr), value :: h

@@ -32,7 +32,7 @@ INFO_NAMES = ["L_LB", "dt", "nu", "kappa", "gbeta", "tauf", "Snu", "Sq",
               "sigk", "H"]
 
 EXPORTS = ["lbm_create", "lbm_destroy", "lbm_get_grid", "lbm_get_info",
-           "lbm_init_rest", "lbm_init_fields", "lbm_step", "lbm_check",
+           "lbm_init_rest", "lbm_init_fields", "lbm_step", "lbm_check", "lbm_check_noalloc",
            "lbm_nusselt", "lbm_get_fields", "lbm_run", "lbm_get_itc",
            "lbm_time_steps", "lbm_synchronize", "lbm_state_size",
            "lbm_save_state", "lbm_load_state", "lbm_copy_state", "lbm_set_launch",
@@ -92,6 +92,7 @@ def load():
             "lbm_step": [H, I64],
             "lbm_check": [H, DP, DP],
             "lbm_nusselt": [H, DP, DP],
+            "lbm_check_noalloc": [H, DP, DP],
             "lbm_get_fields": [H, DP, DP, DP, DP, DP],
             "lbm_run": [H, ctypes.POINTER(I64), DP, DP],
             "lbm_get_itc": [H, ctypes.POINTER(I64)],
