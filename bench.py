@@ -29,6 +29,9 @@ WORKLOADS = {
     # the step count at which the FP32 fields are compared with the FP64 path
     "3d256": dict(dim=3, n=256, Ra=1e5, check_steps=2000,
                   label="cavity3d_256_ra1e5_d3q19_d3q7"),
+    # the reference's own default 3D grid (natural-convection_3dim_mono.f90:6)
+    "3d257": dict(dim=3, n=257, Ra=1e5, check_steps=2000,
+                  label="cavity3d_257_ra1e5_d3q19_d3q7"),
     "2d1025": dict(dim=2, n=1025, Ra=1e6, check_steps=20000,
                    label="cavity2d_1025_ra1e6_d2q9_d2q5"),
     "batch64": dict(dim=2, count=64, seed=20261203,

@@ -207,10 +207,12 @@ int32_t lbm_copy_state(lbm_handle dst, lbm_handle src);
  *    workgroup marches a strip of 64 columns through the grid with the
  *    populations staged in LDS and a register ring:
  *      3D: a 64 x by column of nodes in z; bx = 64, by = 2, 4, 6 or 8,
- *          n % 4 == 0 and n <= 544;
+ *          6 <= n <= 544 (when the far wall would sit in a tile's first
+ *          column or row, that column or row is updated by a second,
+ *          one-node-per-thread launch);
  *      2D: one wave per 64 columns, in y; bx = 64, by = 1.
  *    The variants read the same stencil values and agree to FP32 rounding.
- * Defaults set by lbm_create: 3D FP32 with n % 4 == 0 and n <= 544: variant
+ * Defaults set by lbm_create: 3D FP32 with 6 <= n <= 544: variant
  * 1, 64 x 4, remap 2; 2D with n <= 384: 32 x 2, remap 1; otherwise 64 x 4,
  * remap 1, variant 0. */
 typedef struct lbm_launch {

@@ -143,31 +143,33 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double *__re
 // and, for an FP64 destination, the lagged force F = g beta T rho of the
 // FP64 path from the macroscopic fields of the same step (which the FP32
 // path recovers from the node's own populations).
+// The two storages may pad their population rows differently (spitch,
+// dpitch; plane strides splane, dplane); the fields are unpadded.
 template <int DIM, typename SS, bool SSHIFT, typename DS, bool DSHIFT>
-__global__ __launch_bounds__(256) void k_convert_state(const SS *__restrict__ src,
-                                                       DS *__restrict__ dst, long plane, long nn,
-                                                       int n, const float *__restrict__ tref,
-                                                       const double *__restrict__ rho,
-                                                       const double *__restrict__ T,
-                                                       double *__restrict__ force, double gbeta) {
+__global__ __launch_bounds__(256) void k_convert_state(
+    const SS *__restrict__ src, long splane, int spitch, DS *__restrict__ dst, long dplane,
+    int dpitch, long nn, int n, const float *__restrict__ tref, const double *__restrict__ rho,
+    const double *__restrict__ T, double *__restrict__ force, double gbeta) {
     using FL = Flow<DIM>;
     using HT = Heat<DIM>;
     const long node = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= nn) return;
     const int ih = DIM == 2 ? (int)(node % n) : (int)((node / n) % n);
+    const long row = node / n, col = node - row * n;
+    const long sn = row * spitch + col, dn = row * dpitch + col;
     const double tl = (double)tref[ih];
     static_for<0, FL::Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
         constexpr double wa = FL::w[al];
-        const double f = (double)src[al * plane + node] + (SSHIFT ? wa : 0.0);
-        dst[al * plane + node] = (DS)(DSHIFT ? f - wa : f);
+        const double f = (double)src[al * splane + sn] + (SSHIFT ? wa : 0.0);
+        dst[al * dplane + dn] = (DS)(DSHIFT ? f - wa : f);
     });
     static_for<0, HT::Q>([&](auto A) {
         constexpr int al = decltype(A)::value;
         constexpr double wa = HT::w[al];
         const double ws = wa * tl;
-        const double g = (double)src[(FL::Q + al) * plane + node] + (SSHIFT ? ws : 0.0);
-        dst[(FL::Q + al) * plane + node] = (DS)(DSHIFT ? g - ws : g);
+        const double g = (double)src[(FL::Q + al) * splane + sn] + (SSHIFT ? ws : 0.0);
+        dst[(FL::Q + al) * dplane + dn] = (DS)(DSHIFT ? g - ws : g);
     });
     if (force) force[node] = gbeta * T[node] * rho[node];
 }
@@ -182,12 +184,14 @@ static hipError_t convert_state(const Ctx &dst, const Ctx &src) {
     const double gb = dst.info[LBM_INFO_GBETA];
     if (src.fp64)
         hipLaunchKernelGGL((k_convert_state<DIM, double, false, float, true>), grid, block, 0,
-                           dst.stream, (const double *)s, (float *)d, src.plane, src.nn, src.n,
-                           src.d_tref, src.rho, src.T, force, gb);
+                           dst.stream, (const double *)s, src.plane, src.pitch, (float *)d,
+                           dst.plane, dst.pitch, src.nn, src.n, src.d_tref, src.rho, src.T,
+                           force, gb);
     else
         hipLaunchKernelGGL((k_convert_state<DIM, float, true, double, false>), grid, block, 0,
-                           dst.stream, (const float *)s, (double *)d, src.plane, src.nn, src.n,
-                           src.d_tref, src.rho, src.T, force, gb);
+                           dst.stream, (const float *)s, src.plane, src.pitch, (double *)d,
+                           dst.plane, dst.pitch, src.nn, src.n, src.d_tref, src.rho, src.T,
+                           force, gb);
     return hipGetLastError();
 }
 

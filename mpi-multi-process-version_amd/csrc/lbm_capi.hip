@@ -114,6 +114,7 @@ template <typename A> void set_consts(const Ctx &c, A &a, bool shifted) {
     const double Snu = c.info[LBM_INFO_SNU], Sq = c.info[LBM_INFO_SQ];
     const double dt = c.info[LBM_INFO_DT];
     a.n = c.n;
+    a.pitch = c.pitch;
     a.plane = c.plane;
     a.dt = R(dt);
     a.half_dt = R(0.5 * dt);
@@ -208,6 +209,19 @@ int32_t run_prologue(Ctx &c) {
     c.initialized = true;
     c.inits++;
     return LBM_OK;
+}
+
+// Launch variant 1 (LDS-staged z-march) in 3D: its staging offsets are
+// 32-bit element offsets over the 26 padded planes (n <= 544); any
+// n from 6 up (the nodes its tiles cannot hold are updated by a few
+// workgroups of the same launch with the single-node code, see k_step_zm).
+// Launch variant 1 in 2D (the y-march) loads per lane and has no alignment
+// condition: FP32 storage, bx = 64 and by = 1 (one wave per workgroup).
+bool zm_launch_ok(const lbm_ctx &c, int bx, int by, int occ) {
+    if (c.fp64 || bx != 64 || occ != 0) return false;
+    if (c.dim == 2) return by == 1;
+    return (by == 2 || by == 4 || by == 6 || by == 8) && c.n >= 6 &&
+           26.0 * (double)c.plane < 4294967296.0;
 }
 
 int32_t check_handle(lbm_handle h, bool need_init) {
@@ -319,7 +333,9 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
     {
         // kernels address a population plane with 32-bit byte offsets
         const double esz = p.precision == LBM_PREC_FP64 ? 8.0 : 4.0;
-        const double nodes = p.dim == 2 ? (double)p.nx * p.nx : (double)p.nx * p.nx * p.nx;
+        const double pitch = (p.dim == 3 && p.precision != LBM_PREC_FP64)
+                                 ? (double)((p.nx + 31) / 32 * 32) : (double)p.nx;
+        const double nodes = p.dim == 2 ? pitch * p.nx : pitch * p.nx * p.nx;
         if (nodes * esz >= 4294967296.0)
             return fail(LBM_ERR_ARG, "grid too large for one population plane");
     }
@@ -355,17 +371,21 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
         c->shape.bx = 32;
         c->shape.by = 2;
     }
+    c->fp64 = p.precision == LBM_PREC_FP64;
+    // 3D FP32 population rows start on 128-byte lines (pitch a multiple of
+    // 32 elements): the step kernels' row loads and streaming stores then
+    // cover whole lines whatever n is
+    c->pitch = (c->dim == 3 && !c->fp64) ? (c->n + 31) / 32 * 32 : c->n;
+    c->plane = ((long)c->pitch * c->n * (c->dim == 3 ? c->n : 1) + 63) / 64 * 64;
     // 3D FP32: the LDS z-march (variant 1, 64 x 4, XCD tile bands) wherever
     // its preconditions hold (those of lbm_set_launch); at 256^3 it takes
     // 0.72 ms per step against 1.10 ms for the plain kernel
-    if (p.dim == 3 && p.precision == LBM_PREC_FP32_SHIFTED && c->n % 4 == 0 && c->n <= 544) {
+    if (zm_launch_ok(*c, 64, 4, 0) && c->dim == 3) {
         c->shape.bx = 64;
         c->shape.by = 4;
         c->shape.remap = 2;
         c->shape.variant = 1;
     }
-    c->plane = (c->nn + 63) / 64 * 64;
-    c->fp64 = p.precision == LBM_PREC_FP64;
     build_grid(*c);
     build_params(*c);
     const double tauf = c->info[LBM_INFO_TAUF];
@@ -760,18 +780,6 @@ int32_t lbm_copy_state(lbm_handle dst, lbm_handle src) {
     return LBM_OK;
 }
 
-// Launch variant 1 (LDS-staged z-march) stages rows with 16-byte loads, so
-// rows must be 16-byte aligned (n % 4 == 0); the far-wall node then never
-// sits in the first column or row of a tile and its one-sided stencil stays
-// inside the staged halo.  Its staging offsets are 32-bit element offsets
-// over the 26 planes (26 n^3 < 2^32: n <= 544).
-// Launch variant 1 in 2D (the y-march) loads per lane and has no alignment
-// condition: FP32 storage, bx = 64 and by = 1 (one wave per workgroup).
-static bool zm_launch_ok(const lbm_ctx &c, int bx, int by, int occ) {
-    if (c.fp64 || bx != 64 || occ != 0) return false;
-    if (c.dim == 2) return by == 1;
-    return (by == 2 || by == 4 || by == 6 || by == 8) && c.n % 4 == 0 && c.n <= 544;
-}
 
 int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
     if (!h || !l) return fail(LBM_ERR_ARG, "null argument");
@@ -783,7 +791,8 @@ int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
     if (l->variant == 1 && !zm_launch_ok(*h, l->bx, l->by, l->occupancy))
         return fail(LBM_ERR_ARG,
                     "variant 1 needs FP32 storage, bx = 64, occupancy 0 and, in 2D, by = 1; "
-                    "in 3D, by = 2, 4, 6 or 8, n % 4 == 0 and n <= 544");
+                    "in 3D, by = 2, 4, 6 or 8, n >= 6 and 26 padded population planes "
+                    "within 32-bit element offsets (n <= 544)");
     h->shape.bx = l->bx;
     h->shape.by = l->by;
     h->shape.remap = l->remap;

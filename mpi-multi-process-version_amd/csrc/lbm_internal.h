@@ -27,7 +27,11 @@ struct LaunchShape {
 struct Ctx {
     lbm_params p{};
     int dim = 2, n = 0, qf = 9, qg = 5;
-    long nn = 0, plane = 0;
+    int pitch = 0;              // population row pitch (elements): 3D FP32 storage
+                                // pads rows to a multiple of 32 (128-byte lines);
+                                // otherwise n
+    long nn = 0, plane = 0;     // plane: population plane stride, pitch*n(*n)
+                                // rounded up to 64 elements
     bool fp64 = false;          // FP64 storage and arithmetic
     double info[LBM_INFO_COUNT] = {};
     double dx0 = 0;

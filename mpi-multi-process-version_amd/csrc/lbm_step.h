@@ -25,7 +25,10 @@
 // fields that the residual check, the Nusselt number and the export read.
 //
 // Layout: populations are SoA planes, plane a at p + a*plane, flow planes
-// 0..Qf-1 then thermal planes Qf..Qf+Qg-1; node index i + n*j (+ n*n*k).
+// 0..Qf-1 then thermal planes Qf..Qf+Qg-1; node index i + P*j (+ P*n*k),
+// with the row pitch P = pitch >= n (3D FP32 storage pads rows to 32
+// elements, so that every row starts on a 128-byte line); the FP64
+// macroscopic fields are unpadded, i + n*j (+ n*n*k).
 // Shifted storage (SHIFT = true) keeps f - w_a and g - w_a*Tref(x), where
 // Tref is the linear conduction profile between the heated walls, so FP32
 // resolves the small deviations that carry the flow and the near-wall
@@ -69,6 +72,9 @@ template <typename R, typename S = R> struct StepArgs {
     R *__restrict__ force;  // FP64 path: per-node body force F(k), in place
     long plane;
     int n;
+    int pitch;             // population row pitch in elements (>= n; rows of the
+                           // population planes are padded to it, the FP64
+                           // fields are not)
     int gx, gy, ntiles;    // tile grid of the launch (x, y tiles; all tiles)
     int remap;             // 1: XCD-aware tile order
     const R *__restrict__ tref;  // shifted path: Tref at each heated-axis index
@@ -540,8 +546,9 @@ __device__ __forceinline__ HeatShift<R> load_heat_shift(const StepArgs<R, S> &a,
     return h;
 }
 
+// node: the node's population index; fnode: its field index.
 template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, bool NT = false>
-__device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node,
+__device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node, uint32_t fnode,
                                           const int (&idx)[3], const R *pcf, const R *pcg,
                                           R *f, R *g, R sp_f, R sp_g, const HeatShift<R> &hs) {
     constexpr bool EXACT = std::is_same<S, double>::value;
@@ -596,7 +603,7 @@ __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node
     // zeroth moments of the node's own P(k).
     R F_prev;
     if constexpr (EXACT) {
-        F_prev = ldp(a.force, node);
+        F_prev = ldp(a.force, fnode);
     } else {
         const R rho_prev = SHIFT ? R(1) + sp_f : sp_f;
         const R T_prev = SHIFT ? tloc + sp_g : sp_g;
@@ -662,20 +669,20 @@ __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node
             double dr = 0.0, dT = 0.0;
             static_for<0, QF>([&](auto A) { dr += double(f[decltype(A)::value]); });
             static_for<0, QG>([&](auto A) { dT += double(g[decltype(A)::value]); });
-            a.rho[node] = 1.0 + dr;
-            a.T[node] = double(tloc) + dT;
+            a.rho[fnode] = 1.0 + dr;
+            a.T[fnode] = double(tloc) + dT;
         } else {
-            a.rho[node] = double(rho);
-            a.T[node] = double(T);
+            a.rho[fnode] = double(rho);
+            a.T[fnode] = double(T);
         }
-        a.u[node] = double(ux);
-        a.v[node] = double(uy);
-        if constexpr (DIM == 3) a.w[node] = double(uz);
+        a.u[fnode] = double(ux);
+        a.v[fnode] = double(uy);
+        if constexpr (DIM == 3) a.w[fnode] = double(uz);
     }
 
     // 4. collision with the force of this step
     const R Fb = a.gbeta * T * rho;
-    if constexpr (EXACT) stp<NT>(a.force, node, Fb);
+    if constexpr (EXACT) stp<NT>(a.force, fnode, Fb);
     R fp[QF], gp[QG];
     collide_flow<DIM, R, EXACT>(f, SHIFT ? srho : rho, rho, ux, uy, uz, Fb, a, fp);
     collide_heat<DIM, R, EXACT>(g, SHIFT ? sT : T, T, ux, uy, uz, a, gp);
@@ -700,8 +707,9 @@ __device__ __forceinline__ void step_node(const StepArgs<R, S> &a, int i, int j,
     using HT = Heat<DIM>;
     constexpr int QF = FL::Q, QG = HT::Q;
     const int n = a.n;
-    const uint32_t sy = n, sz = (uint32_t)n * n;
+    const uint32_t sy = a.pitch, sz = (uint32_t)a.pitch * n;
     const uint32_t node = i + sy * j + sz * k;
+    const uint32_t fnode = i + (uint32_t)n * j + (uint32_t)n * n * k;
     const int idx[3] = {i, j, k};
     const uint32_t stride[3] = {1, sy, sz};
     AxisStencil<R> st[3];
@@ -733,7 +741,7 @@ __device__ __forceinline__ void step_node(const StepArgs<R, S> &a, int i, int j,
         g[al] = interp_pop<FL::c[al][0], FL::c[al][1], FL::c[al][2], EXACT>(
             a.pin + (QF + al) * a.plane, node, pcg[al], st, stride);
     });
-    node_tail<DIM, S, R, SHIFT, MACRO, NT>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
+    node_tail<DIM, S, R, SHIFT, MACRO, NT>(a, node, fnode, idx, pcf, pcg, f, g, sp_f, sp_g,
                                            load_heat_shift<DIM, SHIFT>(a, idx));
 }
 
@@ -760,6 +768,9 @@ __global__ __launch_bounds__(256) void k_prologue(StepArgs<R, S> a, StepArgs<dou
     const long nn = DIM == 2 ? (long)a.n * a.n : (long)a.n * a.n * a.n;
     const long node = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= nn) return;
+    // population index of the field node (rows padded to the pitch)
+    const long row = node / a.n;
+    const long pnode = row * a.pitch + (node - row * a.n);
     const double rho = a.rho[node], u = a.u[node], v = a.v[node];
     const double w = DIM == 3 ? a.w[node] : 0.0, T = a.T[node];
     double f[QF], g[QG], fp[QF], gp[QG];
@@ -785,14 +796,14 @@ __global__ __launch_bounds__(256) void k_prologue(StepArgs<R, S> a, StepArgs<dou
     static_for<0, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
         constexpr double wa = FL::w[al];
-        a.pout[al * a.plane + node] = S(SHIFT ? fp[al] - wa : fp[al]);
+        a.pout[al * a.plane + pnode] = S(SHIFT ? fp[al] - wa : fp[al]);
     });
     int hidx = DIM == 2 ? (int)(node % a.n) : (int)((node / a.n) % a.n);
     const double tloc = SHIFT ? double(a.tref[hidx]) : 0.0;
     static_for<0, QG>([&](auto A) {
         constexpr int al = decltype(A)::value;
         constexpr double wa = HT::w[al];
-        a.pout[(QF + al) * a.plane + node] = S(SHIFT ? gp[al] - wa * tloc : gp[al]);
+        a.pout[(QF + al) * a.plane + pnode] = S(SHIFT ? gp[al] - wa * tloc : gp[al]);
     });
 }
 

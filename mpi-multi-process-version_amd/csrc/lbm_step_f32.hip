@@ -72,11 +72,22 @@ int zm_planes_per_group(int n, int by) {
 
 template <int BY>
 static void launch_zm(bool macro, bool nt, dim3 grid, dim3 block, const StepArgs<float> &a, int kc,
-                      hipStream_t s) {
-    if (macro && nt) hipLaunchKernelGGL((k_step_zm<BY, true, true>), grid, block, 0, s, a, kc);
-    else if (macro) hipLaunchKernelGGL((k_step_zm<BY, true, false>), grid, block, 0, s, a, kc);
-    else if (nt) hipLaunchKernelGGL((k_step_zm<BY, false, true>), grid, block, 0, s, a, kc);
-    else hipLaunchKernelGGL((k_step_zm<BY, false, false>), grid, block, 0, s, a, kc);
+                      int xz, int yz, int rim_wgs, hipStream_t s) {
+    if (macro && nt)
+        hipLaunchKernelGGL((k_step_zm<BY, true, true>), grid, block, 0, s, a, kc, xz, yz, rim_wgs);
+    else if (macro)
+        hipLaunchKernelGGL((k_step_zm<BY, true, false>), grid, block, 0, s, a, kc, xz, yz, rim_wgs);
+    else if (nt)
+        hipLaunchKernelGGL((k_step_zm<BY, false, true>), grid, block, 0, s, a, kc, xz, yz, rim_wgs);
+    else
+        hipLaunchKernelGGL((k_step_zm<BY, false, false>), grid, block, 0, s, a, kc, xz, yz,
+                           rim_wgs);
+}
+
+// The region the z-march covers, i < xz and j < yz (k_step_zm's header).
+void zm_region(int n, int by, int &xz, int &yz) {
+    xz = (n % 64 == 1 && n > 64) ? n - 1 : n;
+    yz = ((n - 1) % by == 0) ? n - 1 : n;
 }
 
 hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchShape &ls,
@@ -86,18 +97,30 @@ hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchSha
     StepArgs<float> a = a0;
     const int kc = zm_planes_per_group(a.n, ls.by);
     const int nzc = (a.n + kc - 1) / kc;
-    a.gx = (a.n + 63) / 64;
-    a.gy = (a.n + ls.by - 1) / ls.by;
+    int xz, yz;
+    zm_region(a.n, ls.by, xz, yz);
+    a.gx = (xz + 63) / 64;
+    a.gy = (yz + ls.by - 1) / ls.by;
     // remap 2 splits the tile rows into 8 XCD bands (k_step_zm's zm_tile);
     // every XCD gets the largest band's share of workgroups
     a.remap = (ls.remap == 2 && a.gy < 8) ? 0 : ls.remap;
     a.ntiles = a.remap == 2 ? 8 * a.gx * ((a.gy + 7) / 8) * nzc : a.gx * a.gy * nzc;
-    dim3 block(64, ls.by, 1), grid(a.ntiles);
+    // the nodes outside the marched region: one per thread in the launch's
+    // first workgroups, a multiple of 8 of them so that the march keeps its
+    // workgroup-to-XCD map
+    // the nodes outside the marched region (none when n % 4 == 0): a few
+    // workgroups at the head of the launch, each thread taking several
+    // nodes, and a multiple of 8 of them so that the march keeps its
+    // workgroup-to-XCD map; up to 64, about 8 nodes per thread
+    const long rim = ((long)(a.n - xz) * a.n + (long)(a.n - yz) * xz) * a.n;
+    const long per_wg = 8L * 64 * ls.by;
+    const int rim_wgs = (int)std::min<long>(64, ((rim + per_wg - 1) / per_wg + 7) / 8 * 8);
+    dim3 block(64, ls.by, 1), grid(a.ntiles + rim_wgs);
     const bool nt = stream_stores(step_bytes_f32(3, a.n));
-    if (ls.by == 2) launch_zm<2>(macro, nt, grid, block, a, kc, s);
-    else if (ls.by == 6) launch_zm<6>(macro, nt, grid, block, a, kc, s);
-    else if (ls.by == 8) launch_zm<8>(macro, nt, grid, block, a, kc, s);
-    else launch_zm<4>(macro, nt, grid, block, a, kc, s);
+    if (ls.by == 2) launch_zm<2>(macro, nt, grid, block, a, kc, xz, yz, rim_wgs, s);
+    else if (ls.by == 6) launch_zm<6>(macro, nt, grid, block, a, kc, xz, yz, rim_wgs, s);
+    else if (ls.by == 8) launch_zm<8>(macro, nt, grid, block, a, kc, xz, yz, rim_wgs, s);
+    else launch_zm<4>(macro, nt, grid, block, a, kc, xz, yz, rim_wgs, s);
     return hipGetLastError();
 }
 

@@ -287,8 +287,9 @@ __device__ __forceinline__ void ym_march(const StepArgs<float> &a, float *lds, i
             }
         });
         if (live)
-            node_tail<2, float, float, true, MACRO, NT>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
-                                                        hs);
+            // 2D: unpadded rows, so the population and field indices agree
+            node_tail<2, float, float, true, MACRO, NT>(a, node, node, idx, pcf, pcg, f, g, sp_f,
+                                                        sp_g, hs);
 
         lds_barrier();   // every read of this row's staging is done
         if (more) ym_commit<true>(lds, i0, l, n, cur.xv, cur.xh);

@@ -35,10 +35,19 @@
 // computed once (a plan) and the loads and LDS writes of a plane are
 // straight-line code.
 //
-// Preconditions (checked by the host): FP32 storage, 3D, bx = 64, by = BY,
-// n % 4 == 0 (16-byte aligned rows; the far-wall node then never sits in
-// the first column or row of a tile, so its one-sided stencil stays inside
-// the staged halo) and n <= 544 (32-bit element offsets over 26 planes).
+// Any n from 6 up, as long as the 26 population planes stay within 32-bit
+// element offsets (checked by the host), FP32 storage, 3D, bx = 64,
+// by = BY.  The population rows are padded to a pitch of a multiple of 32
+// elements, so every row starts on a 128-byte line and a lane loads four
+// of its x values with one aligned 16-byte load; a quad clamped at the
+// padded row's end lands in LDS at its true columns.  The march
+// covers i < xz, j < yz: the far-wall node's one-sided stencil reaches two
+// nodes inward, beyond the one-node halo, when it would sit in the first
+// column of a tile (n % 64 == 1), so then the last column is left out
+// (xz = n - 1); likewise the last row when it would be a tile's first row
+// ((n - 1) % BY == 0, yz = n - 1).  The first rim_wgs
+// workgroups of the launch update the nodes left out with the single-node
+// code of k_step (zm_rim_nodes), several per thread.
 #pragma once
 
 #include "lbm_step.h"
@@ -128,9 +137,12 @@ __device__ __forceinline__ ZmPlan<BY, Z> zm_plan(const StepArgs<float> &a, int i
                                                  int w, int l) {
     using L = ZmLayout<BY>;
     const int n = a.n;
-    const uint32_t plane = (uint32_t)a.plane;
+    const uint32_t plane = (uint32_t)a.plane, pitch = (uint32_t)a.pitch;
     const int g = l >> 4, c4 = 4 * (l & 15);
-    const int ii = min(i0 + c4, n - 4);   // n % 4 == 0: groups never straddle the edge
+    // a group past the padded row loads its last four columns (padding, or
+    // nodes i >= n that no live lane reads), into their own LDS columns;
+    // i0 and the pitch are multiples of 32, so the load stays aligned
+    const int ii = min(i0 + c4, (int)pitch - 4);
     ZmPlan<BY, Z> pl;
     static_for<0, L::ninstr(Z)>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
@@ -143,8 +155,8 @@ __device__ __forceinline__ ZmPlan<BY, Z> zm_plan(const StepArgs<float> &a, int i
                 if (R >= cum && R < cum + L::H(p)) {
                     const int rr = R - cum;
                     const int jj = min(max(j0 - L::hy(p) + rr, 0), n - 1);
-                    go = (uint32_t)p * plane + (uint32_t)jj * n + ii;
-                    ld = (uint32_t)(L::off(p) + rr * L::P(p) + L::C0(p) + c4);
+                    go = (uint32_t)p * plane + (uint32_t)jj * pitch + ii;
+                    ld = (uint32_t)(L::off(p) + rr * L::P(p) + L::C0(p) + (ii - i0));
                 }
                 cum += L::H(p);
             }
@@ -164,7 +176,7 @@ __device__ __forceinline__ ZmPlan<BY, Z> zm_plan(const StepArgs<float> &a, int i
             const int jj = j0 - L::hy(p) + rr, ih = side ? i0 + 64 : i0 - 1;
             const bool inside = jj >= 0 && jj < n && ih >= 0 && ih < n;
             if (mine && inside) {
-                pl.hoff = (uint32_t)p * plane + (uint32_t)jj * n + ih;
+                pl.hoff = (uint32_t)p * plane + (uint32_t)jj * pitch + ih;
                 pl.hlds = (uint32_t)(L::off(p) + rr * L::P(p) + (side ? 68 : 3));
             }
             h -= cnt;
@@ -188,7 +200,7 @@ __device__ __forceinline__ void zm_fetch(const StepArgs<float> &a, int kk,
                                          const ZmPlan<BY, Z> &pl, ZmStage<BY, Z> &s) {
     using L = ZmLayout<BY>;
     kk = min(kk, a.n - 1);
-    const size_t po = (size_t)kk * (size_t)a.n * (size_t)a.n;
+    const size_t po = (size_t)kk * (size_t)a.pitch * (size_t)a.n;
     static_for<0, L::ninstr(Z)>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
         s.quad[q] = *reinterpret_cast<const zm_f4 *>(a.pin + (po + pl.goff[q]));
@@ -203,7 +215,7 @@ __device__ __forceinline__ void zm_fetch_purez(const StepArgs<float> &a, uint32_
                                                float *v) {
     using L = ZmLayout<BY>;
     kk = min(kk, a.n - 1);
-    const uint32_t node = col + (uint32_t)kk * (uint32_t)a.n * (uint32_t)a.n;
+    const uint32_t node = col + (uint32_t)kk * (uint32_t)a.pitch * (uint32_t)a.n;
     static_for<0, L::NP>([&](auto Pp) {
         constexpr int p = decltype(Pp)::value;
         if constexpr (L::purez(p)) v[L::pzi(p)] = ldp(a.pin + p * a.plane, node);
@@ -341,8 +353,33 @@ __device__ __forceinline__ bool zm_tile(int P, const StepArgs<float> &a, int nzc
     return tz < nzc;
 }
 
+// The nodes the march leaves out (i >= xz or j >= yz, every k), thread t
+// of `threads` taking nodes t, t + threads, ...: per plane, the columns
+// i >= xz of every row, then the rows j >= yz of the columns i < xz.
+template <bool MACRO, bool NT>
+__device__ __forceinline__ void zm_rim_nodes(const StepArgs<float> &a, int xz, int yz, long t,
+                                             long threads) {
+    const int n = a.n;
+    const long cols = (long)(n - xz) * n, per = cols + (long)(n - yz) * xz;
+    for (; t < per * n; t += threads) {
+        const int k = (int)(t / per);
+        long r = t - (long)k * per;
+        int i, j;
+        if (r < cols) {
+            i = xz + (int)(r % (n - xz));
+            j = (int)(r / (n - xz));
+        } else {
+            r -= cols;
+            i = (int)(r % xz);
+            j = yz + (int)(r / xz);
+        }
+        step_node<3, float, float, true, MACRO, NT>(a, i, j, k);
+    }
+}
+
 template <int BY, bool MACRO, bool NT>
-__global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) {
+__global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc, int xz, int yz,
+                                                     int rim_wgs) {
     using L = ZmLayout<BY>;
     using FL = Flow<3>;
     using HT = Heat<3>;
@@ -351,18 +388,28 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) 
     __shared__ zm_f4 lds4[L::total() / 4];
     float *lds = reinterpret_cast<float *>(lds4);
     const int n = a.n;
+    if ((int)blockIdx.x < rim_wgs) {
+        // the nodes outside the marched region, with the single-node code;
+        // these few workgroups come first and run beside the march
+        zm_rim_nodes<MACRO, NT>(a, xz, yz,
+                                (long)blockIdx.x * (64 * BY) + threadIdx.x + 64 * threadIdx.y,
+                                (long)rim_wgs * 64 * BY);
+        return;
+    }
     int tx, ty, tc;
-    if (!zm_tile((int)blockIdx.x, a, (n + kc - 1) / kc, tx, ty, tc)) return;
+    if (!zm_tile((int)blockIdx.x - rim_wgs, a, (n + kc - 1) / kc, tx, ty, tc)) return;
     const int i0 = tx * 64, j0 = ty * BY;
     const int k0 = tc * kc, k1 = min(n, k0 + kc);
     const int l = threadIdx.x;
     const int r = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one row per wave
-    const bool live = i0 + l < n && j0 + r < n;
-    // lanes past the grid edge stage and compute like the last node and
-    // store nothing (every lane must reach the barriers)
-    const int i = min(i0 + l, n - 1), j = min(j0 + r, n - 1);
-    const uint32_t sy = n, sz2 = (uint32_t)n * n;
+    const bool live = i0 + l < xz && j0 + r < yz;
+    // lanes past the marched region stage and compute like its last node
+    // and store nothing (every lane must reach the barriers)
+    const int i = min(i0 + l, xz - 1), j = min(j0 + r, yz - 1);
+    // population indices (rows padded to the pitch) and field indices
+    const uint32_t sy = a.pitch, sz2 = (uint32_t)a.pitch * n;
     const uint32_t col = i + sy * j;
+    const uint32_t fcol = i + (uint32_t)n * j, fsz = (uint32_t)n * n;
     AxisStencil<float> st[3];
     st[0] = load_stencil(a, i);
     st[1] = load_stencil(a, j);
@@ -470,8 +517,8 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc) 
             }
         });
         if (live)
-            node_tail<3, float, float, true, MACRO, NT>(a, node, idx, pcf, pcg, f, g, sp_f, sp_g,
-                                                        hs);
+            node_tail<3, float, float, true, MACRO, NT>(a, node, fcol + fsz * k, idx, pcf, pcg,
+                                                        f, g, sp_f, sp_g, hs);
 
         lds_barrier();   // every read of this plane's staging is done
         if (more) zm_commit<BY, false>(lds, pin_, np);

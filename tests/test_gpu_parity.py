@@ -313,6 +313,20 @@ def test_tile_orders_give_identical_results(hip, dim, n, bx, by):
     (68, 6, 2),     # partial last tile row (2 of 6 rows), 12 tile rows in bands
     (132, 6, 2),
     (72, 4, 2),     # 18 tile rows in bands of 2 and 3
+    # any n: rows not 16-byte aligned and the nodes the march cannot hold
+    # (the far wall in a tile's first column or row, tiles of 2-3 columns)
+    # updated by the rim launch
+    (17, 4, 0),     # far wall in row 0 of the last tile row: rim row
+    (13, 6, 0),     # the same with 6-row tiles
+    (21, 4, 1),     # partial quad at the x edge, far wall at tile row 0
+    (37, 2, 0),
+    (33, 8, 0),
+    (65, 4, 2),     # far wall in column 0 of a tile: rim column and row
+    (66, 4, 0),     # a 2-column last tile: rim columns
+    (67, 2, 2),     # a 3-column last tile, rim row
+    (70, 6, 0),     # 6-column last tile with a partial quad
+    (131, 8, 2),
+    (257, 4, 2),    # the reference's default 3D grid (3dim_mono.f90:6)
 ])
 def test_lds_zmarch_variant_gives_identical_results(hip, n, by, remap):
     """Variant 1 stages the populations in LDS and marches in z; it uses the
@@ -342,7 +356,9 @@ def test_lds_zmarch_variant_gives_identical_results(hip, n, by, remap):
         assert err <= 1e-6, (name, err)
 
 
-@pytest.mark.parametrize("n,steps,by", [(16, 300, 2), (36, 200, 4), (40, 200, 6)])
+@pytest.mark.parametrize("n,steps,by", [(16, 300, 2), (36, 200, 4), (40, 200, 6),
+                                        (17, 300, 4), (21, 200, 2), (13, 300, 6),
+                                        (33, 150, 8)])
 def test_lds_zmarch_variant_matches_oracle(oracle_mod, hip, n, steps, by):
     """Variant 1 against the FP64 oracle, to the FP32 bar of the plain path."""
     print(f"\n[z-march vs oracle] n={n} steps={steps} by={by}", flush=True)
@@ -420,8 +436,7 @@ def test_lds_zmarch_variant_is_rejected_where_it_cannot_run(hip):
              (dict(dim=3, n=64), (128, 2)),          # bx must be 64
              (dict(dim=3, n=64), (64, 16)),          # by 2, 4, 6 or 8
              (dict(dim=3, n=64), (64, 3)),
-             (dict(dim=3, n=66), (64, 4)),           # rows not 16-byte aligned
-             (dict(dim=3, n=65), (64, 2))]
+             (dict(dim=3, n=5), (64, 2))]            # n >= 6
     for kw, (bx, by) in cases:
         s = hip.Solver(kw["dim"], kw["n"], 1e5, precision=kw.get("precision", hip.PREC_FP32_SHIFTED))
         with pytest.raises(hip.LbmError):
@@ -498,7 +513,8 @@ def test_trajectory_keeps_the_cavity_symmetries(hip, dim, n, prec, launch, steps
 
 @pytest.mark.parametrize("dim,n,prec,want", [
     (3, 32, "fp32", dict(bx=64, by=4, remap=2, variant=1)),
-    (3, 30, "fp32", dict(bx=64, by=4, remap=1, variant=0)),
+    (3, 30, "fp32", dict(bx=64, by=4, remap=2, variant=1)),    # any n
+    (3, 257, "fp32", dict(bx=64, by=4, remap=2, variant=1)),
     (3, 32, "fp64", dict(bx=64, by=4, remap=1, variant=0)),
     (2, 129, "fp32", dict(bx=32, by=2, remap=1, variant=0)),
     (2, 129, "fp64", dict(bx=32, by=2, remap=1, variant=0)),
@@ -516,7 +532,7 @@ def test_default_launch_shapes(hip, dim, n, prec, want):
 
 
 @pytest.mark.parametrize("dim,n,prec", [(2, 33, "fp64"), (2, 65, "fp32"), (3, 32, "fp32"),
-                                        (3, 17, "fp64")])
+                                        (3, 17, "fp64"), (3, 21, "fp32")])
 def test_copy_state_same_precision_continues_exactly(hip, dim, n, prec):
     """lbm_copy_state between two handles of one precision (a plain device
     copy): the copy steps on bit for bit like the original."""
@@ -536,10 +552,11 @@ def test_copy_state_same_precision_continues_exactly(hip, dim, n, prec):
     b.close()
 
 
-@pytest.mark.parametrize("dim,n,steps", [(2, 65, 500), (3, 24, 200)])
+@pytest.mark.parametrize("dim,n,steps", [(2, 65, 500), (3, 24, 200), (3, 21, 200)])
 def test_copy_state_promotes_fp32_to_fp64(hip, dim, n, steps):
     """An FP32 run promoted to FP64 half way tracks a pure FP64 run from the
-    same start to within the FP32 bar."""
+    same start to within the FP32 bar (3D: the FP32 population rows are
+    padded to 32 elements, the FP64 ones are not)."""
     f32 = hip.Solver(dim, n, 1e5)
     f64 = hip.Solver(dim, n, 1e5, precision=hip.PREC_FP64)
     ref = hip.Solver(dim, n, 1e5, precision=hip.PREC_FP64)

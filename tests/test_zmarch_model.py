@@ -91,11 +91,34 @@ def planes_per_group(n, by):
     return (n + nzc - 1) // nzc
 
 
+def region(n, by):
+    """zm_region: the z-march covers i < xz, j < yz; k_step_rim the rest."""
+    xz = n - 1 if (n % 64 == 1 and n > 64) else n
+    yz = n - 1 if (n - 1) % by == 0 else n
+    return xz, yz
+
+
+def rim_nodes(n, xz, yz):
+    """k_step_rim's thread -> node map, replayed."""
+    cols, per = (n - xz) * n, (n - xz) * n + (n - yz) * xz
+    out = []
+    for t in range(per * n):
+        k, r = divmod(t, per)
+        if r < cols:
+            i, j = xz + r % (n - xz), r // (n - xz)
+        else:
+            r -= cols
+            i, j = r % xz, yz + r // xz
+        out.append((i, j, k))
+    return out
+
+
 def tile_map(n, by, remap):
     """Workgroup -> (tx, ty, tz) of zm_tile, with the launcher's grid size."""
     kc = planes_per_group(n, by)
     nzc = (n + kc - 1) // kc
-    gx, gy = (n + 63) // 64, (n + by - 1) // by
+    xz, yz = region(n, by)
+    gx, gy = (xz + 63) // 64, (yz + by - 1) // by
     remap = 0 if (remap == 2 and gy < 8) else remap
     ntiles = 8 * gx * ((gy + 7) // 8) * nzc if remap == 2 else gx * gy * nzc
     out = {}
@@ -131,8 +154,12 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
     the lanes at both tile edges, around the middle and around the last live
     lane of a partial tile; the staging covers every lane."""
     L = Layout(by)
+    xz, yz = region(n, by)
+    pitch = (n + 31) // 32 * 32                  # padded population row
+    pstride = (pitch * n * n + 63) // 64 * 64    # population plane stride
     i0, j0 = tx * 64, ty * by
     k0, k1 = tz * kc, min(n, (tz + 1) * kc)
+    assert xz - i0 >= 1 and yz - j0 >= 1
     lds = {}
     checked = 0
     # halo ownership: every halo value of a phase has a thread
@@ -145,18 +172,20 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
         rl, covered = L.rows[z], set()
         for w, l in itertools.product(range(by), range(64)):
             g, c4 = l >> 4, 4 * (l & 15)
-            ii = min(i0 + c4, n - 4)
+            ii = min(i0 + c4, pitch - 4)
             for q in range(L.ninstr(z)):
                 R = min(4 * (q * by + w) + g, len(rl) - 1)
                 covered.add(R)
                 p, rr = rl[R]
                 jj = min(max(j0 - L.hy[p] + rr, 0), n - 1)
-                base = L.off[p] + rr * L.P[p] + L.C0[p] + c4
+                base = L.off[p] + rr * L.P[p] + L.C0[p] + (ii - i0)
                 lo, hi = L.region(p)
-                g_idx = p * n ** 3 + kk * n * n + jj * n + ii   # population planes of n^3
-                assert lo <= base and base + 3 < hi and 0 <= g_idx and g_idx + 3 < NP * n ** 3
+                g_idx = p * pstride + kk * pitch * n + jj * pitch + ii   # padded rows
+                assert lo <= base and base + 3 < hi and 0 <= g_idx and g_idx + 3 < NP * pstride
+                assert g_idx % 4 == 0 and ii + 3 < pitch     # aligned, inside the padded row
                 for e in range(4):
-                    lds[base + e] = (p, kk, jj, ii + e)
+                    # values past the grid's edge are padding
+                    lds[base + e] = (p, kk, jj, ii + e) if ii + e < n else None
             h = l + 64 * w
             for p in range(NP):
                 if not (staged(p) and zpop(p) == z and L.hx[p]):
@@ -182,15 +211,15 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
             assert got == (p, m, j + dy, i + dx), (p, m, i, j, dx, dy, got)
             checked += 1
 
-    last = min(63, n - 1 - i0)
+    last = min(63, xz - 1 - i0)
     cols = {0, 1, 2, 3, 30, 31, 32, 33, 60, 61, 62, 63, last - 1, last, min(63, last + 1)}
     lanes = [(l, r) for r in range(by) for l in sorted(c for c in cols if 0 <= c < 64)]
     ring = {}   # (lane, p) -> [tag of entry 0, 1, 2]
 
     def ring_in(entry, m):
         for l, r in lanes:
-            live = i0 + l < n and j0 + r < n
-            i, j = min(i0 + l, n - 1), min(j0 + r, n - 1)
+            live = i0 + l < xz and j0 + r < yz
+            i, j = min(i0 + l, xz - 1), min(j0 + r, yz - 1)
             for p in range(NP):
                 if not zpop(p):
                     continue
@@ -220,8 +249,8 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
         zo = stencil(k, n)
         want_planes = (k + zo[0], k, k + zo[1])
         for l, r in lanes:
-            live = i0 + l < n and j0 + r < n
-            i, j = min(i0 + l, n - 1), min(j0 + r, n - 1)
+            live = i0 + l < xz and j0 + r < yz
+            i, j = min(i0 + l, xz - 1), min(j0 + r, yz - 1)
             xo, yo = stencil(i, n), stencil(j, n)
             for p in range(NP):
                 if p in (0, 19):
@@ -267,7 +296,13 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
 
 SHAPES = [(12, 2, 0), (20, 4, 0), (68, 4, 1), (72, 2, 2), (96, 4, 0), (132, 4, 2), (24, 8, 0),
           (136, 8, 2), (24, 6, 0), (68, 6, 2), (132, 6, 2), (72, 4, 2), (16, 2, 0), (36, 4, 0),
-          (40, 6, 0)]
+          (40, 6, 0),
+          # any n: rows not 16-byte aligned, partial quads at the edge, the far
+          # wall in a tile's first column (n % 64 == 1) or row ((n-1) % by == 0),
+          # tiles of 2-3 columns (n % 64 == 2, 3)
+          (6, 2, 0), (7, 4, 0), (9, 4, 0), (13, 6, 0), (17, 4, 0), (21, 4, 1), (33, 8, 0),
+          (37, 2, 0), (65, 4, 2), (66, 4, 0), (67, 2, 2), (70, 6, 0), (129, 4, 2), (131, 8, 2),
+          (257, 4, 2), (257, 6, 2)]
 
 
 @pytest.mark.parametrize("n,by,remap", SHAPES)
@@ -296,3 +331,23 @@ def test_zmarch_model_catches_a_wrong_ring_order():
 def test_tile_map_covers_every_tile_once_on_the_bench_grid():
     for by, remap in ((4, 2), (6, 2), (8, 2), (4, 0), (4, 1)):
         tile_map(256, by, remap)
+
+
+@pytest.mark.parametrize("n,by", [(n, by) for n, by, _ in SHAPES])
+def test_zmarch_and_rim_cover_every_node_once(n, by):
+    """The live lanes of the z-march tiles and the rim kernel's threads
+    together update every node of every plane exactly once; the rim is
+    empty when n % 4 == 0 (even by)."""
+    xz, yz = region(n, by)
+    gx, gy = (xz + 63) // 64, (yz + by - 1) // by
+    inner = {(i, j) for i in range(min(gx * 64, xz)) for j in range(min(gy * by, yz))}
+    assert inner == {(i, j) for i in range(xz) for j in range(yz)}
+    rim = rim_nodes(n, xz, yz)
+    assert len(rim) == len(set(rim))
+    rim_plane = {(i, j) for i, j, k in rim if k == 0}
+    assert {(i, j, k) for i, j in rim_plane for k in range(n)} == set(rim)
+    assert not (rim_plane & inner)
+    assert rim_plane | inner == {(i, j) for i in range(n) for j in range(n)}
+    if n % 4 == 0:
+        assert not rim
+
