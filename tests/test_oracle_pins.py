@@ -551,3 +551,41 @@ def test_steady_fixture_matches_reference_run():
     assert hist[0, 1] == 1.0 and hist[0, 2] == 1.0
     assert abs(float(d["nu_hot"]) - float(d["nu_cold"])) < 1e-6
     assert abs(float(d["nu_hot"]) - 2.243) < 2e-3
+
+
+@pytest.mark.parametrize("dim,n", [(2, 9), (3, 7)])
+def test_force_lag_uses_the_force_of_the_steps_start(oracle_mod, dim, n):
+    """The one-step force lag (2M:476, 2M:733-735; 3M:563, 3M:1004-1005):
+    collision_U forms the body force g beta T rho from the macroscopic fields
+    at the start of the step, and macro_u adds half of it to the momentum
+    after streaming, so the velocity along gravity is
+    (sum_a c_a f_a + F/2) / rho with F from T and rho of the previous step.
+    From a state whose T moves by O(0.1) in one step (random T on a coarse
+    grid at Ra = 1e6), the oracle's velocity matches that form to rounding,
+    and the same form with the new T and rho (no lag) differs by far more
+    than rounding, so removing the lag fails here."""
+    rng = np.random.default_rng(3)
+    o = oracle_mod.Oracle(dim, n, 1e6)
+    shape = o.shape
+    T0 = rng.uniform(0.0, 1.0, shape)
+    u0 = 1e-3 * rng.standard_normal(shape)
+    o.init_fields(u0, None, None, T0)
+    o.step(3)
+    gb = o.params()["gbeta"]
+    T_old, rho_old = o.array("T").copy(), o.array("rho").copy()
+    for stage in ("collision_u", "collision_t", "interpolate", "bounceback_u",
+                  "bounceback_t", "macro_u", "macro_t"):
+        getattr(o, stage)()
+    C = lattice(dim)[0]
+    ax = 1 if dim == 2 else 2            # gravity: +y in 2D, +z in 3D
+    f = o.array("f")
+    j = (f * C[:, ax]).sum(axis=-1)
+    rho, T = o.array("rho"), o.array("T")
+    vel = o.array("v" if dim == 2 else "w")
+    lagged = (j + 0.5 * gb * T_old * rho_old) / rho
+    unlagged = (j + 0.5 * gb * T * rho) / rho
+    scale = np.abs(vel).max()
+    assert np.abs(vel - lagged).max() <= 1e-13 * scale
+    assert np.abs(T - T_old).max() > 1e-2
+    assert np.abs(unlagged - lagged).max() > 1e4 * 1e-13 * scale
+    o.close()
