@@ -40,7 +40,7 @@ WORKLOADS = {
                       label="cavity2d_129_ra1e4_to_steady_state"),
     "warm64": dict(dim=2, n=64, held_out=100, seed=7, weights="warm64.npz", itc_max=600000,
                    label="warm_start_64x64_cavities_Ra1e3-1e6_Pr0.7-7_A1-2.5"),
-    "warm257": dict(dim=2, n=257, held_out=24, seed=7, weights="warm257.npz", itc_max=2000000,
+    "warm257": dict(dim=2, n=257, held_out=100, seed=7, weights="warm257.npz", itc_max=2000000,
                     label="warm_start_257x257_cavities_Ra1e3-1e6_Pr0.7-7_A1-2.5"),
     "warm3d128": dict(dim=3, n=128, held_out=1, cube=dict(Ra=1e5, Pr=0.71, stretch=3.2),
                       weights="warm257.npz", itc_max=1200000, eps=2e-6,
@@ -363,6 +363,31 @@ def run_warm(args, cfg):
     print(json.dumps(out), flush=True)
 
 
+def copy_ceiling(args):
+    """The fastest a plain copy moves the 256^3 FP32 step's bytes (26
+    population planes of 256^3 floats read and 26 written, 3.49 GB) on this
+    GPU: torch's device copy, timed with HIP events over args.steps copies."""
+    import torch
+    n = 26 * 256 ** 3
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(max(1, args.warmup)):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    moved = 2.0 * n * 4
+    print(json.dumps({"metric": "plain device copy of the 256^3 step's bytes", "value":
+                      round(moved / ms / 1e6, 1), "unit": "GB/s", "ms_per_copy": round(ms, 5),
+                      "bytes_moved": moved, "frac_of_8TBs": round(moved / ms / 1e6 / HBM_PEAK_GBS, 4),
+                      "steps": args.steps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,7 +407,14 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (5 times), the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--copy-ceiling", action="store_true",
+                    help="time a plain device copy of the 256^3 step's traffic (1.745 GB "
+                         "read and written): the streaming ceiling the roofline fraction "
+                         "is read against")
     args = ap.parse_args()
+    if args.copy_ceiling:
+        copy_ceiling(args)
+        return
     if args.calibrate_copy:
         import torch
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
