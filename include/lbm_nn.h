@@ -10,16 +10,19 @@
  * result.
  *
  * Network: a convolutional encoder-decoder on a G x G node grid (G a
- * multiple of 8).  Input channels: log10(Ra), Pr, and the node coordinates
- * x/H and y/H (the stretched grid), each mapped affinely onto [-1, 1] over
- * its trained range by the caller; output channels: u/V, v/V and T, with
- * V = sqrt(g beta H) / 4 a quarter of the cavity's free-fall velocity (LB
- * units).
- *   encoder: 3x3 conv 4->16, 3x3 stride-2 convs 16->32->64->64 (ReLU)
+ * multiple of 8) with base width W (16 unless created with
+ * lbm_nn_create_width).  Input channels: log10(Ra), Pr, and the node
+ * coordinates x/H and y/H (the stretched grid), each mapped affinely onto
+ * [-1, 1] over its trained range by the caller; output channels: u/V, v/V
+ * and T, with V = sqrt(g beta H) / 4 a quarter of the cavity's free-fall
+ * velocity (LB units).
+ *   encoder: 3x3 conv 4->W, 3x3 stride-2 convs W->2W->4W->4W (ReLU)
  *   decoder: three stages of x2 nearest upsampling, concatenation with the
  *            encoder feature map of the same size, 3x3 conv (ReLU):
- *            (64+64)->64, (64+32)->32, (32+16)->16
- *   head:    1x1 conv 16->3 (linear)
+ *            (4W+4W)->4W, (4W+2W)->2W, (2W+W)->W
+ *   head:    1x1 conv W->3 (linear)
+ * That is 657 W^2 + 57 W + 3 parameters: 169 107 at W = 16, 516 687 at
+ * W = 28.
  * Every convolution is an im2col product on the matrix cores: BF16 weights
  * and activations, FP32 accumulation; master weights, biases, gradients and
  * the Adam state are FP32.
@@ -46,6 +49,9 @@ typedef struct lbm_nn_ctx *lbm_nn;
  * call may pass; seed: weight initialisation (He-uniform, zero biases). */
 int32_t lbm_nn_create(int32_t grid, int32_t batch_max, uint64_t seed, int32_t device,
                       lbm_nn *out);
+/* The same with base width `width` (a multiple of 4 from 4 to 64). */
+int32_t lbm_nn_create_width(int32_t grid, int32_t width, int32_t batch_max, uint64_t seed,
+                            int32_t device, lbm_nn *out);
 int32_t lbm_nn_destroy(lbm_nn net);
 int32_t lbm_nn_param_count(lbm_nn net, int64_t *count);
 int32_t lbm_nn_get_params(lbm_nn net, float *params);

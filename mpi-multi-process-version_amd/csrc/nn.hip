@@ -419,6 +419,7 @@ struct ConvLayer {
 
 struct lbm_nn_ctx {
     int G = 0, Bmax = 0, device = 0;
+    int W = 16;   // base width: encoder W, 2W, 4W, 4W; decoder 4W, 2W, W channels
     hipStream_t s = nullptr;
     std::vector<ConvLayer> L;
     long np = 0;
@@ -589,7 +590,7 @@ int32_t check_net(lbm_nn n, int32_t batch) {
 // x (host, sample-major) -> y (device, channel-major) through the network
 int32_t forward(lbm_nn n, int B, const float *x_host) {
     const int G = n->G;
-    const long P = (long)G * G;
+    const long P = (long)G * G, W = n->W;
     NN_TRY(hipMemcpyAsync(n->stage, x_host, sizeof(float) * B * LBM_NN_IN_CHANNELS * P,
                           hipMemcpyHostToDevice, n->s));
     hipLaunchKernelGGL(nn::k_to_cmajor, dim3(nn::grid_for(B * 4 * P)), dim3(256), 0, n->s,
@@ -597,22 +598,24 @@ int32_t forward(lbm_nn n, int B, const float *x_host) {
     hipLaunchKernelGGL(nn::k_to_bf16, dim3(nn::grid_for(n->np)), dim3(256), 0, n->s,
                        (const float *)n->w, n->wb, n->np);
     const int G2 = G / 2, G4 = G / 4, G8 = G / 8;
-    float *A1 = n->c7 + 32L * B * P, *A2 = n->c6 + 64L * B * G2 * G2,
-          *A3 = n->c5 + 64L * B * G4 * G4;
+    // the encoder maps a1, a2, a3 sit behind the upsampled decoder maps in
+    // the concatenations c7 = [up(a6), a1], c6 = [up(a5), a2], c5 = [up(a4), a3]
+    float *A1 = n->c7 + 2 * W * B * P, *A2 = n->c6 + 4 * W * B * G2 * G2,
+          *A3 = n->c5 + 4 * W * B * G4 * G4;
     hipError_t e = hipSuccess;
     auto run = [&](hipError_t r) { if (e == hipSuccess) e = r; };
     run(conv_fwd(n, n->L[0], B, n->x0, G, A1));
     run(conv_fwd(n, n->L[1], B, A1, G, A2));
     run(conv_fwd(n, n->L[2], B, A2, G2, A3));
     run(conv_fwd(n, n->L[3], B, A3, G4, n->a4));
-    hipLaunchKernelGGL(nn::k_up2, dim3(nn::grid_for(64L * B * G4 * G4)), dim3(256), 0, n->s,
-                       (const float *)n->a4, n->c5, 64L * B, G8, G8);
+    hipLaunchKernelGGL(nn::k_up2, dim3(nn::grid_for(4 * W * B * G4 * G4)), dim3(256), 0, n->s,
+                       (const float *)n->a4, n->c5, 4 * W * B, G8, G8);
     run(conv_fwd(n, n->L[4], B, n->c5, G4, n->a5));
-    hipLaunchKernelGGL(nn::k_up2, dim3(nn::grid_for(64L * B * G2 * G2)), dim3(256), 0, n->s,
-                       (const float *)n->a5, n->c6, 64L * B, G4, G4);
+    hipLaunchKernelGGL(nn::k_up2, dim3(nn::grid_for(4 * W * B * G2 * G2)), dim3(256), 0, n->s,
+                       (const float *)n->a5, n->c6, 4 * W * B, G4, G4);
     run(conv_fwd(n, n->L[5], B, n->c6, G2, n->a6));
-    hipLaunchKernelGGL(nn::k_up2, dim3(nn::grid_for(32L * B * P)), dim3(256), 0, n->s,
-                       (const float *)n->a6, n->c7, 32L * B, G2, G2);
+    hipLaunchKernelGGL(nn::k_up2, dim3(nn::grid_for(2 * W * B * P)), dim3(256), 0, n->s,
+                       (const float *)n->a6, n->c7, 2 * W * B, G2, G2);
     run(conv_fwd(n, n->L[6], B, n->c7, G, n->a7));
     run(conv_fwd(n, n->L[7], B, n->a7, G, n->y));
     run(hipGetLastError());
@@ -623,29 +626,29 @@ int32_t forward(lbm_nn n, int B, const float *x_host) {
 // loss of y against target (host, sample-major); fills dy and the gradient
 int32_t backward(lbm_nn n, int B, const float *target, float *loss) {
     const int G = n->G, G2 = G / 2, G4 = G / 4, G8 = G / 8;
-    const long P = (long)G * G;
+    const long P = (long)G * G, W = n->W;
     const long ny = 3L * B * P;
     NN_TRY(hipMemcpyAsync(n->stage, target, sizeof(float) * ny, hipMemcpyHostToDevice, n->s));
     hipLaunchKernelGGL(nn::k_to_cmajor, dim3(nn::grid_for(ny)), dim3(256), 0, n->s,
                        (const float *)n->stage, n->tgt, B, 3, P);
     hipLaunchKernelGGL(nn::k_mse, dim3(nn::kMseBlocks), dim3(256), 0, n->s, (const float *)n->y,
                        (const float *)n->tgt, n->dy, n->red, ny);
-    float *A1 = n->c7 + 32L * B * P, *A2 = n->c6 + 64L * B * G2 * G2,
-          *A3 = n->c5 + 64L * B * G4 * G4;
-    float *dA1 = n->dc7 + 32L * B * P, *dA2 = n->dc6 + 64L * B * G2 * G2,
-          *dA3 = n->dc5 + 64L * B * G4 * G4;
+    float *A1 = n->c7 + 2 * W * B * P, *A2 = n->c6 + 4 * W * B * G2 * G2,
+          *A3 = n->c5 + 4 * W * B * G4 * G4;
+    float *dA1 = n->dc7 + 2 * W * B * P, *dA2 = n->dc6 + 4 * W * B * G2 * G2,
+          *dA3 = n->dc5 + 4 * W * B * G4 * G4;
     hipError_t e = hipSuccess;
     auto run = [&](hipError_t r) { if (e == hipSuccess) e = r; };
     run(conv_bwd(n, n->L[7], B, n->a7, G, n->y, n->dy, n->da7, false));
     run(conv_bwd(n, n->L[6], B, n->c7, G, n->a7, n->da7, n->dc7, false));
-    hipLaunchKernelGGL(nn::k_up2_bwd, dim3(nn::grid_for(32L * B * G2 * G2)), dim3(256), 0, n->s,
-                       (const float *)n->dc7, n->da6, 32L * B, G2, G2);
+    hipLaunchKernelGGL(nn::k_up2_bwd, dim3(nn::grid_for(2 * W * B * G2 * G2)), dim3(256), 0,
+                       n->s, (const float *)n->dc7, n->da6, 2 * W * B, G2, G2);
     run(conv_bwd(n, n->L[5], B, n->c6, G2, n->a6, n->da6, n->dc6, false));
-    hipLaunchKernelGGL(nn::k_up2_bwd, dim3(nn::grid_for(64L * B * G4 * G4)), dim3(256), 0, n->s,
-                       (const float *)n->dc6, n->da5, 64L * B, G4, G4);
+    hipLaunchKernelGGL(nn::k_up2_bwd, dim3(nn::grid_for(4 * W * B * G4 * G4)), dim3(256), 0,
+                       n->s, (const float *)n->dc6, n->da5, 4 * W * B, G4, G4);
     run(conv_bwd(n, n->L[4], B, n->c5, G4, n->a5, n->da5, n->dc5, false));
-    hipLaunchKernelGGL(nn::k_up2_bwd, dim3(nn::grid_for(64L * B * G8 * G8)), dim3(256), 0, n->s,
-                       (const float *)n->dc5, n->da4, 64L * B, G8, G8);
+    hipLaunchKernelGGL(nn::k_up2_bwd, dim3(nn::grid_for(4 * W * B * G8 * G8)), dim3(256), 0,
+                       n->s, (const float *)n->dc5, n->da4, 4 * W * B, G8, G8);
     run(conv_bwd(n, n->L[3], B, A3, G4, n->a4, n->da4, dA3, true));
     run(conv_bwd(n, n->L[2], B, A2, G2, A3, dA3, dA2, true));
     run(conv_bwd(n, n->L[1], B, A1, G, A2, dA2, dA1, true));
@@ -668,19 +671,29 @@ extern "C" {
 
 int32_t lbm_nn_create(int32_t grid, int32_t batch_max, uint64_t seed, int32_t device,
                       lbm_nn *out) {
+    return lbm_nn_create_width(grid, 16, batch_max, seed, device, out);
+}
+
+int32_t lbm_nn_create_width(int32_t grid, int32_t width, int32_t batch_max, uint64_t seed,
+                            int32_t device, lbm_nn *out) {
     if (!out) return lbm::set_error(LBM_ERR_ARG, "null output");
     *out = nullptr;
     if (grid < 16 || grid % 8 || batch_max < 1 || (long)batch_max * grid * grid > (1L << 26))
         return lbm::set_error(LBM_ERR_ARG, "grid must be a multiple of 8 (>= 16), batch >= 1");
+    if (width < 4 || width > 64 || width % 4)
+        return lbm::set_error(LBM_ERR_ARG, "width must be a multiple of 4 from 4 to 64");
     lbm_nn n = new (std::nothrow) lbm_nn_ctx();
     if (!n) return lbm::set_error(LBM_ERR_NOMEM, "network");
     n->G = grid;
+    n->W = width;
     n->Bmax = batch_max;
     n->device = device;
     // layers: cin, cout, k, stride, relu
-    const int spec[8][5] = {{4, 16, 3, 1, 1},  {16, 32, 3, 2, 1}, {32, 64, 3, 2, 1},
-                            {64, 64, 3, 2, 1}, {128, 64, 3, 1, 1}, {96, 32, 3, 1, 1},
-                            {48, 16, 3, 1, 1}, {16, 3, 1, 1, 0}};
+    const int W = width;
+    const int spec[8][5] = {{4, W, 3, 1, 1},         {W, 2 * W, 3, 2, 1},
+                            {2 * W, 4 * W, 3, 2, 1}, {4 * W, 4 * W, 3, 2, 1},
+                            {8 * W, 4 * W, 3, 1, 1}, {6 * W, 2 * W, 3, 1, 1},
+                            {3 * W, W, 3, 1, 1},     {W, 3, 1, 1, 0}};
     long off = 0;
     for (auto &sp : spec) {
         ConvLayer l{sp[0], sp[1], sp[2], sp[3], sp[4], 0, 0};
@@ -714,11 +727,14 @@ int32_t lbm_nn_create(int32_t grid, int32_t batch_max, uint64_t seed, int32_t de
     int32_t rc = LBM_OK;
     auto A = [&](auto *&p, long cnt) { if (!rc) rc = dalloc(n, p, cnt); };
     A(n->w, n->np); A(n->g, n->np); A(n->m, n->np); A(n->v, n->np); A(n->wb, n->np);
-    A(n->x0, 4 * B * P); A(n->c7, 48 * B * P); A(n->c6, 96 * B * P2); A(n->c5, 128 * B * P4);
-    A(n->a4, 64 * B * P8); A(n->a5, 64 * B * P4); A(n->a6, 32 * B * P2); A(n->a7, 16 * B * P);
+    const long Wl = W;
+    A(n->x0, 4 * B * P); A(n->c7, 3 * Wl * B * P); A(n->c6, 6 * Wl * B * P2);
+    A(n->c5, 8 * Wl * B * P4); A(n->a4, 4 * Wl * B * P8); A(n->a5, 4 * Wl * B * P4);
+    A(n->a6, 2 * Wl * B * P2); A(n->a7, Wl * B * P);
     A(n->y, 3 * B * P);
-    A(n->dc7, 48 * B * P); A(n->dc6, 96 * B * P2); A(n->dc5, 128 * B * P4);
-    A(n->da4, 64 * B * P8); A(n->da5, 64 * B * P4); A(n->da6, 32 * B * P2); A(n->da7, 16 * B * P);
+    A(n->dc7, 3 * Wl * B * P); A(n->dc6, 6 * Wl * B * P2); A(n->dc5, 8 * Wl * B * P4);
+    A(n->da4, 4 * Wl * B * P8); A(n->da5, 4 * Wl * B * P4); A(n->da6, 2 * Wl * B * P2);
+    A(n->da7, Wl * B * P);
     A(n->dy, 3 * B * P);
     A(n->col, colmax); A(n->dzb, zmax); A(n->dcol, colmax);
     A(n->part, kPartFloats); A(n->red, nn::kMseBlocks);

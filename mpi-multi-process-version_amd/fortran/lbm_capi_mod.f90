@@ -231,6 +231,13 @@ module lbm_capi
             integer(c_int64_t), value :: seed
             type(c_ptr), intent(out) :: net
         end function
+        integer(c_int32_t) function lbm_nn_create_width(grid, width, batch_max, seed, device, &
+                net) bind(C, name='lbm_nn_create_width')
+            import :: c_int32_t, c_int64_t, c_ptr
+            integer(c_int32_t), value :: grid, width, batch_max, device
+            integer(c_int64_t), value :: seed
+            type(c_ptr), intent(out) :: net
+        end function
         integer(c_int32_t) function lbm_nn_destroy(net) bind(C, name='lbm_nn_destroy')
             import :: c_int32_t, c_ptr
             type(c_ptr), value :: net

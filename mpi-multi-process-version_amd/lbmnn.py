@@ -13,7 +13,8 @@ import numpy as np
 
 import lbmhip
 
-EXPORTS = ["lbm_nn_create", "lbm_nn_destroy", "lbm_nn_param_count", "lbm_nn_get_params",
+EXPORTS = ["lbm_nn_create", "lbm_nn_create_width", "lbm_nn_destroy", "lbm_nn_param_count",
+           "lbm_nn_get_params",
            "lbm_nn_set_params", "lbm_nn_forward", "lbm_nn_loss_grad", "lbm_nn_train_step"]
 
 _sig_done = False
@@ -28,6 +29,7 @@ def _lib():
         I32, I64 = ctypes.c_int32, ctypes.c_int64
         sig = {
             "lbm_nn_create": [I32, I32, ctypes.c_uint64, I32, ctypes.POINTER(H)],
+            "lbm_nn_create_width": [I32, I32, I32, ctypes.c_uint64, I32, ctypes.POINTER(H)],
             "lbm_nn_destroy": [H],
             "lbm_nn_param_count": [H, ctypes.POINTER(I64)],
             "lbm_nn_get_params": [H, FP],
@@ -53,13 +55,14 @@ def _f32(a):
 
 
 class Net:
-    """The encoder-decoder on a G x G grid."""
+    """The encoder-decoder on a G x G grid with base width `width`."""
 
-    def __init__(self, grid=64, batch_max=32, seed=1, device=0):
+    def __init__(self, grid=64, batch_max=32, seed=1, device=0, width=16):
         self._L = _lib()
         self.h = ctypes.c_void_p()
-        lbmhip._check(self._L.lbm_nn_create(grid, batch_max, seed, device, ctypes.byref(self.h)))
-        self.grid, self.batch_max = grid, batch_max
+        lbmhip._check(self._L.lbm_nn_create_width(grid, width, batch_max, seed, device,
+                                                  ctypes.byref(self.h)))
+        self.grid, self.batch_max, self.width = grid, batch_max, width
 
     def close(self):
         if self.h:

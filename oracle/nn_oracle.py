@@ -1,4 +1,5 @@
-"""numpy restatement of the warm-start network (include/lbm_nn.h).
+"""numpy restatement of the warm-start network (include/lbm_nn.h), at any
+base width W (the width follows from the parameter vector's size).
 
 TEST INFRASTRUCTURE ONLY: the checker of the HIP network.  Only tests/ may
 import this module.
@@ -14,18 +15,33 @@ everything is FP64, which the finite-difference tests use.
 """
 import numpy as np
 
-LAYERS = [(4, 16, 3, 1, True), (16, 32, 3, 2, True), (32, 64, 3, 2, True),
-          (64, 64, 3, 2, True), (128, 64, 3, 1, True), (96, 32, 3, 1, True),
-          (48, 16, 3, 1, True), (16, 3, 1, 1, False)]
+def layers(width=16):
+    """(cin, cout, kernel, stride, relu) of the 8 convolutions at base width W."""
+    W = width
+    return [(4, W, 3, 1, True), (W, 2 * W, 3, 2, True), (2 * W, 4 * W, 3, 2, True),
+            (4 * W, 4 * W, 3, 2, True), (8 * W, 4 * W, 3, 1, True), (6 * W, 2 * W, 3, 1, True),
+            (3 * W, W, 3, 1, True), (W, 3, 1, 1, False)]
 
 
-def param_count():
-    return sum(co * ci * k * k + co for ci, co, k, _, _ in LAYERS)
+LAYERS = layers(16)
+
+
+def param_count(width=16):
+    return sum(co * ci * k * k + co for ci, co, k, _, _ in layers(width))
+
+
+def width_of(p):
+    """The base width of a flat parameter vector (its size fixes it)."""
+    n = np.asarray(p).size
+    for w in range(4, 257):
+        if param_count(w) == n:
+            return w
+    raise ValueError(f"no width has {n} parameters")
 
 
 def split_params(p):
     out, o = [], 0
-    for ci, co, k, s, relu in LAYERS:
+    for ci, co, k, s, relu in layers(width_of(p)):
         nw = co * ci * k * k
         out.append((p[o:o + nw].reshape(co, ci, k, k), p[o + nw:o + nw + co]))
         o += nw + co
@@ -115,26 +131,28 @@ def forward(p, x, quant=True, keep=False):
 def loss_grad(p, x, t, quant=True):
     """Mean squared error over all outputs and its parameter gradient."""
     P = split_params(np.asarray(p, dtype=np.float64))
+    W = width_of(p)
+    L = layers(W)
     y, A = forward(p, x, quant, keep=True)
     d = y - np.asarray(t, dtype=np.float64)
     loss = float(np.mean(d * d))
     dy = 2.0 * d / d.size
-    grads = [None] * len(LAYERS)
+    grads = [None] * len(L)
     mask = lambda dA, a: dA * (a > 0)
 
     def bwd(i, dZ, X, need_dx=True):
-        W, _ = P[i]
-        dW, db, dX = conv_bwd(dZ, X, W, LAYERS[i][3], quant, need_dx)
+        Wt, _ = P[i]
+        dW, db, dX = conv_bwd(dZ, X, Wt, L[i][3], quant, need_dx)
         grads[i] = (dW, db)
         return dX
 
     da7 = bwd(7, dy, A["a7"])
     dc7 = bwd(6, mask(da7, A["a7"]), A["c7"])
-    da6, da1 = up2_bwd(dc7[:, :32]), dc7[:, 32:]
+    da6, da1 = up2_bwd(dc7[:, :2 * W]), dc7[:, 2 * W:]
     dc6 = bwd(5, mask(da6, A["a6"]), A["c6"])
-    da5, da2 = up2_bwd(dc6[:, :64]), dc6[:, 64:]
+    da5, da2 = up2_bwd(dc6[:, :4 * W]), dc6[:, 4 * W:]
     dc5 = bwd(4, mask(da5, A["a5"]), A["c5"])
-    da4, da3 = up2_bwd(dc5[:, :64]), dc5[:, 64:]
+    da4, da3 = up2_bwd(dc5[:, :4 * W]), dc5[:, 4 * W:]
     da3 = da3 + bwd(3, mask(da4, A["a4"]), A["a3"])
     da2 = da2 + bwd(2, mask(da3, A["a3"]), A["a2"])
     da1 = da1 + bwd(1, mask(da2, A["a2"]), A["a1"])

@@ -33,12 +33,12 @@ def _data(G, B, seed):
     return x, t
 
 
-@pytest.mark.parametrize("G,B", [(16, 3), (32, 2)])
-def test_forward_matches_oracle(nnmod, G, B):
-    print(f"\n[nn forward] G={G} B={B}", flush=True)
-    net = nnmod.Net(grid=G, batch_max=4, seed=7)
+@pytest.mark.parametrize("G,B,W", [(16, 3, 16), (32, 2, 16), (16, 2, 8), (32, 2, 28)])
+def test_forward_matches_oracle(nnmod, G, B, W):
+    print(f"\n[nn forward] G={G} B={B} width={W}", flush=True)
+    net = nnmod.Net(grid=G, batch_max=4, seed=7, width=W)
     p = net.get_params()
-    assert p.size == O.param_count()
+    assert p.size == O.param_count(W) == 657 * W * W + 57 * W + 3
     x, _ = _data(G, B, 1)
     y = net.forward(x)
     ref = O.forward(p, x, quant=True)
@@ -50,10 +50,11 @@ def test_forward_matches_oracle(nnmod, G, B):
     net.close()
 
 
-def test_gradient_matches_oracle(nnmod):
-    print("\n[nn gradient] G=16 B=2", flush=True)
+@pytest.mark.parametrize("W", [16, 8, 28])
+def test_gradient_matches_oracle(nnmod, W):
+    print(f"\n[nn gradient] G=16 B=2 width={W}", flush=True)
     G, B = 16, 2
-    net = nnmod.Net(grid=G, batch_max=2, seed=3)
+    net = nnmod.Net(grid=G, batch_max=2, seed=3, width=W)
     p = net.get_params()
     x, t = _data(G, B, 2)
     loss, g = net.loss_grad(x, t)
@@ -64,7 +65,7 @@ def test_gradient_matches_oracle(nnmod):
     assert rel <= 1e-2
     # per layer, so that no layer hides behind a large one
     o = 0
-    for ci, co, k, _, _ in O.LAYERS:
+    for ci, co, k, _, _ in O.layers(W):
         n = co * ci * k * k + co
         lr = np.linalg.norm(g[o:o + n] - rg[o:o + n]) / np.linalg.norm(rg[o:o + n])
         assert lr <= 3e-2, (ci, co, lr)

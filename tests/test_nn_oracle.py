@@ -63,14 +63,17 @@ def test_upsample_adjoint():
     assert abs(np.sum(O.up2(x) * y) - np.sum(x * O.up2_bwd(y))) < 1e-12
 
 
-def test_loss_gradient_matches_finite_differences():
+@pytest.mark.parametrize("width", [16, 8])
+def test_loss_gradient_matches_finite_differences(width):
     """Directional derivatives of the FP64 loss along random directions, and
-    single-parameter derivatives of every layer's first weight and bias."""
+    single-parameter derivatives of every layer's first weight and bias, at
+    the default base width and at another one (the concatenations split at
+    2W and 4W channels)."""
     rng = np.random.default_rng(11)
     G, B = 16, 2
-    p = np.zeros(O.param_count())
+    p = np.zeros(O.param_count(width))
     o = 0
-    for ci, co, k, _, _ in O.LAYERS:
+    for ci, co, k, _, _ in O.layers(width):
         nw = co * ci * k * k
         p[o:o + nw] = rng.uniform(-1, 1, nw) * np.sqrt(6.0 / (ci * k * k))
         p[o + nw:o + nw + co] = 0.05 * rng.standard_normal(co)
@@ -78,13 +81,18 @@ def test_loss_gradient_matches_finite_differences():
     x = rng.standard_normal((B, 4, G, G))
     t = rng.standard_normal((B, 3, G, G))
     loss, g = O.loss_grad(p, x, t, quant=False)
-    h = 1e-6
+    # A step along a random direction moves every pre-activation, and at
+    # h = 1e-6 some ReLU input of the width-8 net changes sign inside the
+    # difference interval; h = 1e-7 stays clear of every kink and its FP64
+    # rounding (about 1e-16 |loss| / h) is far below the tolerance.
+    h = 1e-7
     for _ in range(3):
         d = rng.standard_normal(p.size)
         num = (O.loss_grad(p + h * d, x, t, False)[0] - O.loss_grad(p - h * d, x, t, False)[0]) / (2 * h)
         assert abs(num - g @ d) <= 1e-6 * max(1.0, abs(num)), (num, g @ d)
+    h = 1e-6
     o = 0
-    for ci, co, k, _, _ in O.LAYERS:
+    for ci, co, k, _, _ in O.layers(width):
         for idx in (o, o + co * ci * k * k):   # first weight, first bias
             e = np.zeros(p.size)
             e[idx] = 1.0
@@ -112,6 +120,15 @@ def test_header_declares_the_binding_list(hip):
     for n in names:
         assert hasattr(lib, n), n
     assert O.param_count() == 169107
+
+
+def test_parameter_count_closed_form():
+    """657 W^2 + 57 W + 3 parameters at base width W (the layer list of
+    include/lbm_nn.h); the width is recovered from a vector's size."""
+    for w in (4, 8, 16, 24, 28, 32):
+        assert O.param_count(w) == 657 * w * w + 57 * w + 3
+        assert O.width_of(np.zeros(O.param_count(w))) == w
+    assert O.param_count(28) == 516687
 
 
 def test_adam_closed_forms():
