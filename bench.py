@@ -271,13 +271,27 @@ def run_steady(args, cfg):
     print(json.dumps(out), flush=True)
 
 
+def _training_data(record):
+    """One line on the steady fields a network was trained on, from the
+    record surrogate.save_weights() stored with it."""
+    parts = record.get("data")
+    if not parts:
+        return f"{record.get('count')} seeded {record.get('n', 64)}^2 cavities"
+    n = sum(p.get("kept", 0) for p in parts)
+    sizes = sorted({p.get("n") for p in parts})
+    text = f"{n} seeded " + "/".join(f"{k}^2" for k in sizes) + " cavities"
+    if record.get("init"):
+        text += f", starting from {record['init']}"
+    return text
+
+
 def run_warm(args, cfg):
     """The learned warm start against the cold start on held-out cavities:
     each cavity runs from rest and from the trained network's prediction to
     the same stopping rule; the value is the ratio of the wall times until
     the hot-wall Nusselt number stays within 0.1 % of the cold run's
     converged value (network inference and initialisation included).
-    Cavities larger than the network's 64 x 64 grid get its output
+    Cavities on grids other than the network's own get its output
     resampled onto their nodes; the 3D cube gets the 2D prediction of its
     (y, z) cross-section extruded along x.  The cube's rule is eps = 2e-6:
     at 128^3 the FP32 velocity residual wanders between 1e-6 and 2e-6 for
@@ -287,6 +301,7 @@ def run_warm(args, cfg):
     import surrogate
     weights = os.path.join(os.path.dirname(surrogate.WEIGHTS), cfg["weights"])
     net = surrogate.load_net(weights)
+    net_grid, net_width, net_params = net.grid, net.width, net.param_count
     with np.load(weights, allow_pickle=False) as z:
         record = json.loads(str(z["record"]))
     dim, n = cfg["dim"], cfg["n"]
@@ -315,13 +330,12 @@ def run_warm(args, cfg):
         "ms_per_step": round(1e3 * solver_s / steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: seeded held-out cavity parameters; network trained on the "
-                "solver's own steady fields ({})".format(
-                    "4096 seeded 64x64 cavities" if cfg["weights"] == "warm64.npz" else
-                    "the 64x64 network fine-tuned on seeded 257x257 cavities"),
+                "solver's own steady fields ({})".format(_training_data(record)),
         "config": {"workload": cfg["label"], "held_out": len(cavs), "seed": cfg.get("seed"),
                    "eps": eps, "check_every": surrogate.CHECK_EVERY,
                    "itc_max": cfg["itc_max"],
-                   "network": {"weights": "data/" + cfg["weights"],
+                   "network": {"weights": "data/" + cfg["weights"], "grid": net_grid,
+                               "width": net_width, "params": net_params,
                                "train_samples": record.get("train_samples"),
                                "epochs": record.get("epochs"),
                                "val_rms": record.get("final", {}).get("val_rms")},

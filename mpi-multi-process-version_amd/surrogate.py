@@ -22,9 +22,9 @@ and a cube the resulting plane extruded along its adiabatic x axis
 nodes the same way (generate).
 
 Command line (prints one JSON record per stage):
-    python surrogate.py generate --out data.npz --n 257 [--count 256 ...]
+    python surrogate.py generate --out data.npz --n 257 --grid 128 [--count 256 ...]
     python surrogate.py train --out weights.npz [--count 2048 ...]
-    python surrogate.py train --out w.npz --data data.npz [--init warm64.npz]
+    python surrogate.py train --out w.npz --data data.npz [--width 28 | --init warm64.npz]
     python surrogate.py compare --weights weights.npz [--n 257 --count 100 ...]
 """
 import argparse
@@ -43,12 +43,16 @@ import lbmhip   # noqa: E402
 import lbmnn    # noqa: E402
 
 GRID = 64
+WIDTH = 16
 LOG_RA = (3.0, 6.0)
 PR = (0.7, 7.0)
 STRETCH = (1.0, 2.5)
 EPS = 1e-6
 CHECK_EVERY = 2000
 WEIGHTS = os.path.join(_HERE, "data", "warm64.npz")
+# Targets in data files: int16 multiples of 2^-13 (|y| < 4, rounding <= 6.1e-5,
+# far below the network's error).
+TARGET_STEP = 2.0 ** -13
 
 
 def sample_cavities(count, seed, n=GRID, log_ra=LOG_RA, pr=PR, stretch=STRETCH):
@@ -79,21 +83,35 @@ def net_inputs(cavity, s, G=GRID):
     return lbmnn.inputs(cavity["Ra"], cavity["Pr"], xc, Hc), xc / Hc
 
 
-def network_inputs(batch, cavities):
+def network_inputs(batch, cavities, G=GRID):
     """x[b][4][G][G] of the cavities of a Batch and their node lists (see
     net_inputs)."""
-    got = [net_inputs(c, batch[i]) for i, c in enumerate(cavities)]
+    got = [net_inputs(c, batch[i], G) for i, c in enumerate(cavities)]
     return np.stack([g[0] for g in got]), [g[1] for g in got]
 
 
-def solve(cavities, eps=EPS, itc_max=600000, net=None, keep_fields=False):
+def inputs_of(params, G):
+    """Network inputs x[k][4][G][G] of cavities given as rows (Ra, Pr,
+    stretch) on grids other than G x G (net_inputs' erf nodes of G)."""
+    out = np.zeros((len(params), 4, G, G), np.float32)
+    for i, (ra, pr, a) in enumerate(np.asarray(params, np.float64)):
+        xc, Hc = lbmnn.erf_nodes(G, min(max(a, STRETCH[0]), STRETCH[1]))
+        out[i] = lbmnn.inputs(ra, pr, xc, Hc)
+    return out
+
+
+def solve(cavities, eps=EPS, itc_max=600000, net=None, keep_fields=False, G=GRID):
     """Runs every cavity to the stopping rule, from rest (net None) or from
     the network's prediction.  Returns a dict of per-cavity arrays (itc,
     status, errU, errT, Nu on the hot wall) and wall times in seconds:
-    `predict` (inputs, forward, lbm_init_fields), `run` (the time loop)."""
+    `predict` (inputs, forward, lbm_init_fields), `run` (the time loop).
+    G is the grid of the network inputs and kept fields (net.grid when a
+    network is given)."""
+    if net is not None:
+        G = net.grid
     b = _batch(cavities, eps, itc_max)
     t0 = time.perf_counter()
-    x, nodes = network_inputs(b, cavities)
+    x, nodes = network_inputs(b, cavities, G)
     if net is None:
         b.init_rest()
     else:
@@ -110,7 +128,7 @@ def solve(cavities, eps=EPS, itc_max=600000, net=None, keep_fields=False):
     out = dict(itc=itc, status=st, errU=eu, errT=et, nu=np.array(b.nusselt()[0]),
                predict_s=t1 - t0, run_s=t2 - t1, x=x)
     if keep_fields:
-        t = np.zeros((len(cavities), 3, GRID, GRID), dtype=np.float32)
+        t = np.zeros((len(cavities), 3, G, G), dtype=np.float32)
         for i in range(len(cavities)):
             _, u, v, _, T = b[i].fields()
             ti = lbmnn.targets(u, v, T, lbmnn.velocity_scale(b[i].info()))
@@ -129,19 +147,20 @@ def predict(net, x):
 
 
 def generate(count, seed, eps=EPS, itc_max=600000, per_batch=256, keep_tol=1e-4, n=GRID,
-             stretch=STRETCH, log=print):
+             stretch=STRETCH, G=GRID, log=print):
     """Steady fields of `count` seeded cavities on n x n grids, as network
     inputs and targets on the network's G x G nodes (sampled there from a
     larger grid, see net_inputs).  A cavity is kept if it reached the
     stopping rule, or ran to itc_max with errU and errT below keep_tol (a
     slowly settling cavity, already steady to that tolerance); diverged
-    cavities are dropped.  Returns (x, y, record)."""
+    cavities are dropped.  Returns (x, y, record); record["cavities"] lists
+    (Ra, Pr, stretch) of the kept cavities."""
     cavs = sample_cavities(count, seed, n=n, stretch=stretch)
     xs, ys, keep, itcs = [], [], [], []
     t0 = time.perf_counter()
     for s in range(0, count, per_batch):
         part = cavs[s:s + per_batch]
-        r = solve(part, eps, itc_max, keep_fields=True)
+        r = solve(part, eps, itc_max, keep_fields=True, G=G)
         ok = (r["status"] == lbmhip.LBM_OK) | ((r["status"] == lbmhip.LBM_ERR_NOT_CONVERGED) &
                                               (r["errU"] <= keep_tol) & (r["errT"] <= keep_tol))
         xs.append(r["x"][ok])
@@ -153,19 +172,26 @@ def generate(count, seed, eps=EPS, itc_max=600000, per_batch=256, keep_tol=1e-4,
                         "run_s": round(r["run_s"], 2)}))
     itcs = np.concatenate(itcs)
     rec = dict(count=count, seed=seed, kept=len(keep), eps=eps, itc_max=itc_max, keep_tol=keep_tol,
-               itc_median=float(np.median(itcs)), generate_seconds=time.perf_counter() - t0)
+               itc_median=float(np.median(itcs)), generate_seconds=time.perf_counter() - t0,
+               cavities=[[cavs[i]["Ra"], cavs[i]["Pr"], cavs[i]["stretch"]] for i in keep])
     return np.concatenate(xs), np.concatenate(ys), rec
 
 
-def train(net, x, y, epochs, batch=32, lr=2e-3, lr_min=2e-5, seed=0, val=None, log=print):
+def train(net, x, y, epochs, batch=32, lr=2e-3, lr_min=2e-5, seed=0, val=None, log=print,
+          time_budget=None):
     """Adam over shuffled mini-batches with a cosine learning-rate decay;
-    returns the per-epoch mean training loss (and validation loss)."""
+    returns the per-epoch mean training loss (and validation loss).  With
+    a time_budget (seconds), the first epoch's duration caps the number of
+    epochs so that training ends within the budget; the decay then spans
+    the capped count."""
     rng = np.random.default_rng(seed)
     hist = []
-    steps = epochs * ((len(x) + batch - 1) // batch)
+    per_epoch = (len(x) + batch - 1) // batch
+    steps = epochs * per_epoch
     k = 0
     t0 = time.perf_counter()
-    for e in range(epochs):
+    e = 0
+    while e < epochs:
         order = rng.permutation(len(x))
         tot = 0.0
         for s in range(0, len(x), batch):
@@ -173,6 +199,11 @@ def train(net, x, y, epochs, batch=32, lr=2e-3, lr_min=2e-5, seed=0, val=None, l
             rate = lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * k / steps))
             tot += net.train_step(x[idx], y[idx], rate) * len(idx)
             k += 1
+        if e == 0 and time_budget:
+            fit = max(2, int(time_budget / (time.perf_counter() - t0)))
+            if fit < epochs:
+                epochs, steps = fit, fit * per_epoch
+                log(json.dumps(dict(stage="train", epochs_capped_to=fit)))
         entry = {"epoch": e + 1, "loss": tot / len(x)}
         if val is not None and ((e + 1) % 10 == 0 or e + 1 == epochs):
             d = predict(net, val[0]) - val[1]
@@ -182,19 +213,22 @@ def train(net, x, y, epochs, batch=32, lr=2e-3, lr_min=2e-5, seed=0, val=None, l
         if (e + 1) % 10 == 0 or e + 1 == epochs:
             log(json.dumps(dict(stage="train", seconds=round(time.perf_counter() - t0, 1),
                                 **entry)))
+        e += 1
     return hist
 
 
 def save_weights(path, net, record):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     np.savez(path, params=net.get_params(), grid=np.int32(net.grid),
-             record=np.array(json.dumps(record)))
+             width=np.int32(net.width), record=np.array(json.dumps(record)))
 
 
 def load_net(path=WEIGHTS, batch_max=32, device=0):
-    """A network with the parameters saved by save_weights()."""
+    """A network with the parameters saved by save_weights() (files without
+    a width hold the base width 16)."""
     with np.load(path, allow_pickle=False) as z:
-        net = lbmnn.Net(grid=int(z["grid"]), batch_max=batch_max, device=device)
+        width = int(z["width"]) if "width" in z.files else WIDTH
+        net = lbmnn.Net(grid=int(z["grid"]), batch_max=batch_max, device=device, width=width)
         net.set_params(z["params"])
     return net
 
@@ -329,9 +363,19 @@ def summarise(rows):
         cold_band_s=tot("cold_band_s", banded), warm_band_s=tot("warm_band_s", banded))
 
 
-def save_data(path, x, y, record):
+def save_data(path, y, record):
+    """Targets y[k][3][G][G] as int16 multiples of TARGET_STEP and the
+    kept cavities' (Ra, Pr, stretch), from which load_data() rebuilds the
+    inputs."""
+    y = np.asarray(y, np.float64)
+    if not np.all(np.abs(y) < 32767 * TARGET_STEP):
+        raise ValueError("target outside the int16 range of save_data")
+    rec = dict(record)
+    cav = np.asarray(rec.pop("cavities"), np.float64)
+    assert len(cav) == len(y)
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-    np.savez(path, x=x, y=y, record=np.array(json.dumps(record)))
+    np.savez_compressed(path, yq=np.rint(y / TARGET_STEP).astype(np.int16), cavities=cav,
+                        record=np.array(json.dumps(rec)))
 
 
 def load_data(paths):
@@ -339,8 +383,9 @@ def load_data(paths):
     xs, ys, recs = [], [], []
     for p in paths:
         with np.load(p, allow_pickle=False) as z:
-            xs.append(z["x"])
-            ys.append(z["y"])
+            y = z["yq"].astype(np.float32) * np.float32(TARGET_STEP)
+            xs.append(inputs_of(z["cavities"], y.shape[-1]))
+            ys.append(y)
             recs.append(json.loads(str(z["record"])))
     return np.concatenate(xs), np.concatenate(ys), recs
 
@@ -351,6 +396,7 @@ def main(argv=None):
     g = sub.add_parser("generate")
     g.add_argument("--out", required=True)
     g.add_argument("--n", type=int, default=GRID)
+    g.add_argument("--grid", type=int, default=GRID, help="network grid of the kept fields")
     g.add_argument("--count", type=int, default=256)
     g.add_argument("--seed", type=int, default=20261203)
     g.add_argument("--itc-max", type=int, default=600000)
@@ -363,6 +409,7 @@ def main(argv=None):
     t.add_argument("--data", nargs="*", default=None,
                    help="train on these generate files instead of generating")
     t.add_argument("--init", default=None, help="start from these weights")
+    t.add_argument("--width", type=int, default=WIDTH, help="base width of a new network")
     t.add_argument("--n", type=int, default=GRID)
     t.add_argument("--count", type=int, default=2048)
     t.add_argument("--seed", type=int, default=20261203)
@@ -372,6 +419,8 @@ def main(argv=None):
     t.add_argument("--itc-max", type=int, default=600000)
     t.add_argument("--per-batch", type=int, default=256)
     t.add_argument("--keep-tol", type=float, default=1e-4)
+    t.add_argument("--time-budget", type=float, default=None,
+                   help="seconds of training; caps the epochs after the first")
     c = sub.add_parser("compare")
     c.add_argument("--weights", default=WEIGHTS)
     c.add_argument("--n", type=int, default=GRID)
@@ -382,10 +431,12 @@ def main(argv=None):
     log = lambda s: print(s, flush=True)
     if a.cmd == "generate":
         x, y, rec = generate(a.count, a.seed, itc_max=a.itc_max, per_batch=a.per_batch,
-                             keep_tol=a.keep_tol, n=a.n, stretch=tuple(a.stretch), log=log)
-        rec.update(n=a.n, stretch=a.stretch)
-        save_data(a.out, x, y, rec)
-        log(json.dumps(dict(stage="saved", path=a.out, **rec)))
+                             keep_tol=a.keep_tol, n=a.n, stretch=tuple(a.stretch), G=a.grid,
+                             log=log)
+        rec.update(n=a.n, stretch=a.stretch, grid=a.grid)
+        save_data(a.out, y, rec)
+        log(json.dumps(dict(stage="saved", path=a.out,
+                            **{k: v for k, v in rec.items() if k != "cavities"})))
     elif a.cmd == "train":
         if a.data:
             x, y, recs = load_data(a.data)
@@ -393,17 +444,19 @@ def main(argv=None):
         else:
             x, y, rec = generate(a.count, a.seed, itc_max=a.itc_max, per_batch=a.per_batch,
                                  keep_tol=a.keep_tol, n=a.n, log=log)
+            rec.pop("cavities")
             rec.update(n=a.n)
         nval = max(1, len(x) // 16)
         if a.init:
             net = load_net(a.init, batch_max=a.batch)
             rec.update(init=os.path.basename(a.init))
         else:
-            net = lbmnn.Net(grid=GRID, batch_max=a.batch, seed=a.seed)
+            net = lbmnn.Net(grid=x.shape[-1], batch_max=a.batch, seed=a.seed, width=a.width)
         t0 = time.perf_counter()
         hist = train(net, x[nval:], y[nval:], a.epochs, a.batch, a.lr,
-                     val=(x[:nval], y[:nval]), log=log)
-        rec.update(epochs=a.epochs, batch=a.batch, lr=a.lr, train_samples=len(x) - nval,
+                     val=(x[:nval], y[:nval]), log=log, time_budget=a.time_budget)
+        rec.update(epochs=len(hist), grid=net.grid, width=net.width, params=net.param_count,
+                   batch=a.batch, lr=a.lr, train_samples=len(x) - nval,
                    val_samples=nval, final=hist[-1], train_seconds=time.perf_counter() - t0)
         save_weights(a.out, net, rec)
         log(json.dumps(dict(stage="saved", path=a.out, **rec)))

@@ -139,3 +139,24 @@ def test_summary_counts_only_cavities_the_cold_run_judged():
     assert np.isclose(s["band_itc_ratio"], 40000 / 8000)
     assert np.isclose(s["nu_rel_dev_max"], 3e-4)
     assert S.summarise([dict(cold_status=lbmhip.LBM_ERR_DIVERGED)])["judged"] == 0
+
+
+def test_data_files_round_trip(tmp_path):
+    """save_data keeps targets to half an int16 step (2^-14) and the kept
+    cavities' parameters; load_data rebuilds the inputs net_inputs() gives
+    a cavity on a grid other than the network's."""
+    rng = np.random.default_rng(4)
+    G = 16
+    cav = S.sample_cavities(3, 11, n=257)
+    y = rng.uniform(-3.9, 3.9, (3, 3, G, G)).astype(np.float32)
+    rec = dict(count=3, seed=11, cavities=[[c["Ra"], c["Pr"], c["stretch"]] for c in cav])
+    S.save_data(str(tmp_path / "d.npz"), y, rec)
+    x2, y2, recs = S.load_data([str(tmp_path / "d.npz")])
+    assert np.abs(y2 - y).max() <= 2.0 ** -14 * (1 + 1e-6)
+    assert recs == [dict(count=3, seed=11)]
+    for i, c in enumerate(cav):
+        xc, Hc = lbmnn.erf_nodes(G, c["stretch"])
+        np.testing.assert_array_equal(x2[i], lbmnn.inputs(c["Ra"], c["Pr"], xc, Hc))
+    import pytest
+    with pytest.raises(ValueError):
+        S.save_data(str(tmp_path / "e.npz"), np.full((1, 3, G, G), 4.0), dict(cavities=[[1e4, 1, 2]]))
