@@ -291,9 +291,16 @@ int32_t lbm_batch_active(lbm_batch b, int32_t *cavities, int64_t *nodes);
 int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms);
 
 /* Block shape of the batched launch (bx a multiple of 32, bx*by a multiple
- * of 64 and at most 1024; remap and occupancy must be 0; variant 0, or 1 for
- * the 2D y-march with bx = 64, by = 1), and its autotuner, which tries both
- * variants (restores every cavity's state). */
+ * of 64 and at most 1024; remap and occupancy must be 0), and its
+ * autotuner, which tries every variant the batch can take (restores every
+ * cavity's state).  Variants:
+ *  0: one launch per step, one node per thread, tiles of every cavity;
+ *  1: one launch per step, the 2D y-march (bx = 64, by = 1);
+ *  2: reserved (a persistent work-queue launch, switched off: rejected);
+ *  3: resident cavities (bx = 512, by = 1; every cavity's edge at most 70):
+ *     one workgroup per cavity holds the cavity's populations in registers
+ *     and LDS and runs all the steps of an lbm_batch_step call (up to 4096
+ *     per launch) in one launch; results equal variant 0's to the bit. */
 int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *launch);
 int32_t lbm_batch_get_launch(lbm_batch b, lbm_launch *launch);
 int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms);

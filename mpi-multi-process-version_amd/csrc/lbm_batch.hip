@@ -8,7 +8,10 @@
 // arguments from a device array (two per cavity, one per ping-pong parity)
 // and runs the single-cavity node update on it.  Tiles are listed largest
 // cavity first, so the big cavities start early and the small ones fill
-// the tail of the launch.
+// the tail of the launch.  Batches of small cavities (edge <= 70) can
+// instead keep every cavity on the chip (variant 3, lbm_step_res.h): one
+// workgroup per cavity holds its state in registers and LDS and runs all
+// the steps of an lbm_batch_step call in one launch.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -42,7 +45,8 @@ struct lbm_batch_ctx {
     // (73.8 against 76.9) and 64 of 257^2 (137.8 against 145.7; 128 x 4:
     // 132.4)
     int bx = 64, by = 8;
-    int variant = 0;   // 0: one node per thread; 1: y-march strips; 2: persistent
+    int variant = 0;   // 0: one node per thread; 1: y-march strips; 2: persistent;
+                       // 3: resident (one workgroup per cavity)
     int kc = 0;        // variant 1: rows per chunk
     // variant 2 (one persistent launch per lbm_batch_step call): per-cavity
     // (first table entry, gx, gy), the MALL group offsets, the ticket
@@ -72,6 +76,10 @@ namespace {
 // not established, so lbm_batch_set_launch refuses the variant and the
 // autotuner skips it.
 constexpr bool kPersistentEnabled = false;
+
+// Variant 3 runs at most this many steps per launch (a launch then lasts
+// tens of milliseconds even for the largest resident cavities).
+constexpr int64_t kResStepsPerLaunch = 4096;
 
 // bytes a 2D FP32 step moves per node: 14 populations read and written
 constexpr double kStepBytesPerNode = 14 * 4.0 * 2.0;
@@ -142,6 +150,14 @@ int32_t need_active(lbm_batch b) {
     return set_error(LBM_ERR_STATE, "no cavity to step: initialise one (lbm_batch_init_rest, or "
                                     "lbm_init_rest/lbm_init_fields/lbm_load_state on "
                                     "lbm_batch_cavity) or re-initialise a retired one");
+}
+
+// Variant 3 holds each cavity in one workgroup: every cavity's edge must be
+// at most lbm::res_max_edge().
+bool resident_fits(const lbm_batch b) {
+    for (lbm_handle h : b->cav)
+        if (h->n > lbm::res_max_edge()) return false;
+    return true;
 }
 
 bool table_stale(const lbm_batch b) {
@@ -246,7 +262,9 @@ int32_t rebuild(lbm_batch b) {
         if (!b->active[c]) continue;
         const lbm_ctx &h = *b->cav[c];
         b->base_cur[c] = h.cur;
-        if (b->variant == 1) {
+        if (b->variant == 3) {
+            tiles.push_back(make_int4(c, 0, 0, 2 * c + h.cur));
+        } else if (b->variant == 1) {
             const int gx = (h.n + 63) / 64, nch = (h.n + b->kc - 1) / b->kc;
             for (int ch = 0; ch < nch; ch++)
                 for (int tx = 0; tx < gx; tx++)
@@ -309,6 +327,26 @@ int32_t batch_steps(lbm_batch b, int64_t nsteps) {
     for (size_t c = 0; c < b->cav.size(); c++)
         if (b->active[c]) nodes += (double)b->cav[c]->nn;
     const int ng = (int)b->groups.size() - 1;
+    if (b->variant == 3) {
+        // every cavity in one workgroup for up to kResStepsPerLaunch steps
+        // per launch; the launch's last step writes the macroscopic fields
+        int max_n = 0;
+        for (size_t c = 0; c < b->cav.size(); c++)
+            if (b->active[c]) max_n = std::max(max_n, b->cav[c]->n);
+        for (int64_t s0 = 0; s0 < nsteps; s0 += kResStepsPerLaunch) {
+            const int64_t d = std::min(kResStepsPerLaunch, nsteps - s0);
+            hipError_t e = lbm::launch_step_batch_res_f32(b->d_args, b->d_tiles, b->ntiles,
+                                                          b->parity, (int)d, max_n, b->stream);
+            if (e != hipSuccess) return hip_err(e, "resident batch launch");
+            b->parity ^= (int)(d & 1);
+        }
+        for (size_t c = 0; c < b->cav.size(); c++) {
+            if (!b->active[c]) continue;
+            b->cav[c]->cur ^= (int)(nsteps & 1);
+            b->cav[c]->itc += nsteps;
+        }
+        return LBM_OK;
+    }
     if (b->variant == 2) {
         // one persistent launch for all nsteps (several only if the ticket
         // count would pass 2^31)
@@ -619,10 +657,14 @@ int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms) {
 int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *l) {
     if (!b || !l) return set_error(LBM_ERR_ARG, "null argument");
     if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
-        l->remap != 0 || l->occupancy != 0 || l->variant < 0 || l->variant > 2 ||
-        (l->variant == 1 && (l->bx != 64 || l->by != 1)))
+        l->remap != 0 || l->occupancy != 0 || l->variant < 0 || l->variant > 3 ||
+        (l->variant == 1 && (l->bx != 64 || l->by != 1)) ||
+        (l->variant == 3 && (l->bx != 512 || l->by != 1)))
         return set_error(LBM_ERR_ARG, "invalid batched launch configuration (variant 1: bx = 64, "
-                                      "by = 1)");
+                                      "by = 1; variant 3: bx = 512, by = 1)");
+    if (l->variant == 3 && !resident_fits(b))
+        return set_error(LBM_ERR_ARG, "variant 3 (resident cavities) needs every cavity's edge "
+                                      "to be at most " + std::to_string(lbm::res_max_edge()));
     if (l->variant == 2 && !kPersistentEnabled)
         return set_error(LBM_ERR_ARG, "variant 2 (the persistent launch) is switched off: its "
                                       "first GPU runs stopped making progress and the cause is "
@@ -656,16 +698,17 @@ int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms
         if ((rc = lbm_save_state(b->cav[c], saved[c].data(), bytes))) return rc;
     }
     // (bx, by, variant); variant 1 is the y-march, variant 2 the
-    // persistent launch
+    // persistent launch, variant 3 the resident cavities
     static const int cand[][3] = {{64, 1, 0},  {64, 2, 0},  {64, 4, 0},  {64, 8, 0},
                                   {128, 1, 0}, {128, 2, 0}, {128, 4, 0}, {256, 1, 0},
                                   {256, 2, 0}, {32, 8, 0},  {64, 1, 1},  {64, 4, 2},
-                                  {64, 8, 2},  {128, 4, 2}, {32, 8, 2}};
+                                  {64, 8, 2},  {128, 4, 2}, {32, 8, 2},  {512, 1, 3}};
     const int bx0 = b->bx, by0 = b->by, v0 = b->variant;
     int best_bx = bx0, best_by = by0, best_v = v0;
     double best = 1e300;
     for (const auto &cd : cand) {
         if (cd[2] == 2 && !kPersistentEnabled) continue;
+        if (cd[2] == 3 && !resident_fits(b)) continue;
         b->bx = cd[0];
         b->by = cd[1];
         b->variant = cd[2];

@@ -121,6 +121,15 @@ hipError_t launch_step_batch_ym_f32(bool macro, const StepArgs<float> *args, con
 // Rows per chunk of the y-march for a total of strip_rows (strips x rows).
 int ym_rows_per_chunk(double strip_rows);
 
+// The batch's variant 3: one launch of `steps` time steps of every cavity
+// in table[0..ncav) (table[b].w ^ parity = argument index of cavity b reading
+// its current buffer), one workgroup per cavity holding the cavity in
+// registers and LDS (lbm_step_res.h); max_n = the largest edge, at most
+// res_max_edge().
+hipError_t launch_step_batch_res_f32(const StepArgs<float> *args, const int4 *table, int ncav,
+                                     int parity, int steps, int max_n, hipStream_t s);
+int res_max_edge();
+
 // The batch's variant 2: one persistent launch of `steps` time steps over
 // the variant-0 tile table (tiles[e] = (cavity, tile x, tile y, argument
 // index at parity 0), each cavity's tiles contiguous, y-major), cut into

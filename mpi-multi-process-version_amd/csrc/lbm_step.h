@@ -523,11 +523,6 @@ __device__ __forceinline__ void tile_coords(int P, int G, int gx, int gy, int re
     tz = r / gy;
 }
 
-// Steps 2-4 of a node once f(k), g(k) are interpolated: the thermal shift
-// correction, the wall rules (with the node's own P(k) in pcf, pcg), the
-// macroscopic fields with the lagged force, the collision and the stores.
-// sp_f, sp_g are the sums of the node's own P(k) (the zeroth moments that
-// carry rho(k-1), T(k-1) on the FP32 path).
 // The shifted path's per-node constants along the heated axis: Tref at the
 // node and the streaming corrections for c = +1 and c = -1.
 template <typename R> struct HeatShift {
@@ -546,18 +541,40 @@ __device__ __forceinline__ HeatShift<R> load_heat_shift(const StepArgs<R, S> &a,
     return h;
 }
 
-// node: the node's population index; fnode: its field index.
-template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, bool NT = false>
-__device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node, uint32_t fnode,
-                                          const int (&idx)[3], const R *pcf, const R *pcg,
-                                          R *f, R *g, R sp_f, R sp_g, const HeatShift<R> &hs) {
-    constexpr bool EXACT = std::is_same<S, double>::value;
+// Step 2 for the flow populations: the wall rules at wall nodes
+// (half-way bounce-back, 2dim_mono.f90:662-696, 3dim_mono.f90:878-943),
+// with the node's own P(k) in pcf.
+template <int DIM, typename R>
+__device__ __forceinline__ void node_walls_flow(int n, const int (&idx)[3], const R *pcf, R *f) {
+    using FL = Flow<DIM>;
+    constexpr int QF = FL::Q;
+    static_for<0, DIM>([&](auto AX) {
+        constexpr int ax = decltype(AX)::value;
+        const int side = idx[ax] == 0 ? 1 : (idx[ax] == n - 1 ? -1 : 0);
+        if (side != 0) {
+            static_for<1, QF>([&](auto A) {
+                constexpr int al = decltype(A)::value;
+                constexpr int ca = FL::c[al][ax];
+                if constexpr (ca != 0) {
+                    if (ca * side > 0) f[al] = pcf[opposite<DIM>(al)];
+                }
+            });
+        }
+    });
+}
+
+// Step 2 for the thermal populations: the shifted path's streaming
+// correction, then the wall rules (hot/cold anti-bounce-back on the heated
+// axis, adiabatic bounce-back elsewhere; 2dim_mono.f90:698-724,
+// 3dim_mono.f90:945-986) with the node's own P(k) in pcg.
+template <int DIM, bool SHIFT, typename R, typename S>
+__device__ __forceinline__ void node_walls_heat(const StepArgs<R, S> &a, const int (&idx)[3],
+                                                const R *pcg, R *g, const HeatShift<R> &hs) {
     using FL = Flow<DIM>;
     using HT = Heat<DIM>;
-    constexpr int QF = FL::Q, QG = HT::Q;
+    constexpr int QG = HT::Q;
     const int n = a.n;
     constexpr int HAX = heated_axis<DIM>();
-    const R tloc = hs.tloc;
     if constexpr (SHIFT) {
         static_for<1, QG>([&](auto A) {
             constexpr int al = decltype(A)::value;
@@ -571,13 +588,6 @@ __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node
         constexpr int ax = decltype(AX)::value;
         const int side = idx[ax] == 0 ? 1 : (idx[ax] == n - 1 ? -1 : 0);
         if (side != 0) {
-            static_for<1, QF>([&](auto A) {
-                constexpr int al = decltype(A)::value;
-                constexpr int ca = FL::c[al][ax];
-                if constexpr (ca != 0) {
-                    if (ca * side > 0) f[al] = pcf[opposite<DIM>(al)];
-                }
-            });
             static_for<1, QG>([&](auto A) {
                 constexpr int al = decltype(A)::value;
                 constexpr int ca = FL::c[al][ax];
@@ -596,20 +606,29 @@ __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node
             });
         }
     });
+}
 
-    // 3. macroscopic fields of this step and the force of the previous one.
-    // The FP64 path keeps F(k) per node, as the reference does (Fx/Fy/Fz,
-    // 2dim_mono.f90:47); the FP32 path recovers it from the conserved
-    // zeroth moments of the node's own P(k).
-    R F_prev;
-    if constexpr (EXACT) {
-        F_prev = ldp(a.force, fnode);
-    } else {
-        const R rho_prev = SHIFT ? R(1) + sp_f : sp_f;
-        const R T_prev = SHIFT ? tloc + sp_g : sp_g;
-        F_prev = a.gbeta * T_prev * rho_prev;
-    }
+// The lagged force F(k) of the FP32 path from the sums of the node's own
+// P(k), i.e. from rho(k-1) and T(k-1), which the collision conserves.
+template <bool SHIFT, typename R>
+__device__ __forceinline__ R lagged_force(R gbeta, R sp_f, R sp_g, R tloc) {
+    const R rho_prev = SHIFT ? R(1) + sp_f : sp_f;
+    const R T_prev = SHIFT ? tloc + sp_g : sp_g;
+    return gbeta * T_prev * rho_prev;
+}
 
+// Steps 3 and 4 of a node: the macroscopic fields of this step with the
+// lagged force F_prev (2dim_mono.f90:726-753), written to the FP64 fields
+// on MACRO steps, and the collision with the force of this step
+// (2dim_mono.f90:431-569), giving the node's P(k+1) in fp, gp.  fnode is
+// the node's field index.
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, bool NT = false>
+__device__ __forceinline__ void node_collide(const StepArgs<R, S> &a, uint32_t fnode, const R *f,
+                                             const R *g, R F_prev, R tloc, R *fp, R *gp) {
+    constexpr bool EXACT = std::is_same<S, double>::value;
+    using FL = Flow<DIM>;
+    using HT = Heat<DIM>;
+    constexpr int QF = FL::Q, QG = HT::Q;
     R srho = R(0), jx = R(0), jy = R(0), jz = R(0), sT = R(0);
     if constexpr (EXACT) {
         static_for<0, QF>([&](auto A) { srho += f[decltype(A)::value]; });
@@ -683,9 +702,75 @@ __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node
     // 4. collision with the force of this step
     const R Fb = a.gbeta * T * rho;
     if constexpr (EXACT) stp<NT>(a.force, fnode, Fb);
-    R fp[QF], gp[QG];
     collide_flow<DIM, R, EXACT>(f, SHIFT ? srho : rho, rho, ux, uy, uz, Fb, a, fp);
     collide_heat<DIM, R, EXACT>(g, SHIFT ? sT : T, T, ux, uy, uz, a, gp);
+}
+
+// Steps 2-4 of a node once f(k), g(k) are interpolated: the thermal shift
+// correction, the wall rules (with the node's own P(k) in pcf, pcg), the
+// macroscopic fields with the lagged force, the collision and the stores.
+// sp_f, sp_g are the sums of the node's own P(k) (the zeroth moments that
+// carry rho(k-1), T(k-1) on the FP32 path).  node: the node's population
+// index; fnode: its field index.
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, bool NT = false>
+__device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node, uint32_t fnode,
+                                          const int (&idx)[3], const R *pcf, const R *pcg,
+                                          R *f, R *g, R sp_f, R sp_g, const HeatShift<R> &hs) {
+    constexpr bool EXACT = std::is_same<S, double>::value;
+    using FL = Flow<DIM>;
+    using HT = Heat<DIM>;
+    constexpr int QF = FL::Q, QG = HT::Q;
+    const int n = a.n;
+    constexpr int HAX = heated_axis<DIM>();
+    // node_walls_heat and node_walls_flow, interleaved per axis (the
+    // instruction order the compiler schedules with the fewest registers)
+    if constexpr (SHIFT) {
+        static_for<1, QG>([&](auto A) {
+            constexpr int al = decltype(A)::value;
+            if constexpr (FL::c[al][HAX] != 0) {
+                constexpr double wa = HT::w[al];
+                g[al] += R(wa) * (FL::c[al][HAX] > 0 ? hs.tc_plus : hs.tc_minus);
+            }
+        });
+    }
+    static_for<0, DIM>([&](auto AX) {
+        constexpr int ax = decltype(AX)::value;
+        const int side = idx[ax] == 0 ? 1 : (idx[ax] == n - 1 ? -1 : 0);
+        if (side != 0) {
+            static_for<1, QF>([&](auto A) {
+                constexpr int al = decltype(A)::value;
+                constexpr int ca = FL::c[al][ax];
+                if constexpr (ca != 0) {
+                    if (ca * side > 0) f[al] = pcf[opposite<DIM>(al)];
+                }
+            });
+            static_for<1, QG>([&](auto A) {
+                constexpr int al = decltype(A)::value;
+                constexpr int ca = FL::c[al][ax];
+                if constexpr (ca != 0) {
+                    if (ca * side > 0) {
+                        R gpo = pcg[opposite<DIM>(al)];
+                        if constexpr (ax == heated_axis<DIM>()) {
+                            constexpr double w2 = 2.0 * HT::w[al];
+                            R tw = side > 0 ? a.tw_hot : a.tw_cold;
+                            g[al] = -gpo + R(w2) * tw;
+                        } else {
+                            g[al] = gpo;
+                        }
+                    }
+                }
+            });
+        }
+    });
+    // 3. the force of the previous step.  The FP64 path keeps F(k) per
+    // node, as the reference does (Fx/Fy/Fz, 2dim_mono.f90:47); the FP32
+    // path recovers it from the conserved zeroth moments of the node's own
+    // P(k).
+    R F_prev;
+    if constexpr (EXACT) F_prev = ldp(a.force, fnode);
+    else F_prev = lagged_force<SHIFT>(a.gbeta, sp_f, sp_g, hs.tloc);
+    R fp[QF], gp[QG];
+    node_collide<DIM, S, R, SHIFT, MACRO, NT>(a, fnode, f, g, F_prev, hs.tloc, fp, gp);
     static_for<0, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
         stp<NT>(a.pout + al * a.plane, node, S(fp[al]));

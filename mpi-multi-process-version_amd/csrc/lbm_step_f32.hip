@@ -1,6 +1,7 @@
 // lbm_step_f32.hip -- FP32 shifted-population instantiation of the fused
 // step (the production path).  Built with default FMA contraction.
 #include "lbm_internal.h"
+#include "lbm_step_res.h"
 #include "lbm_step_ym.h"
 #include "lbm_step_zm.h"
 
@@ -193,6 +194,34 @@ hipError_t launch_step_batch_f32(bool macro, const StepArgs<float> *args, const 
         hipLaunchKernelGGL((k_step_batch<false, true>), grid, block, 0, s, args, tiles, parity);
     else
         hipLaunchKernelGGL((k_step_batch<false, false>), grid, block, 0, s, args, tiles, parity);
+    return hipGetLastError();
+}
+
+// The batch's variant 3 (lbm_step_res.h): one workgroup per cavity, steps
+// time steps per launch.  Nodes per thread are instantiated for K = 1, 2,
+// 4, 6, 8, 9 and 10 (64^2 cavities take 8, 65^2 take 9, 70^2 take 10); the
+// launch uses the smallest K that covers every listed cavity.
+template <int K>
+static void launch_res(const StepArgs<float> *args, const int4 *table, int ncav, int parity,
+                       int steps, hipStream_t s) {
+    hipLaunchKernelGGL((k_step_batch_res<K>), dim3(ncav), dim3(kResThreads), 0, s, args, table,
+                       parity, steps);
+}
+
+int res_max_edge() { return kResMaxN; }
+
+hipError_t launch_step_batch_res_f32(const StepArgs<float> *args, const int4 *table, int ncav,
+                                     int parity, int steps, int max_n, hipStream_t s) {
+    if (ncav <= 0 || steps <= 0) return hipSuccess;
+    if (max_n < 3 || max_n > kResMaxN) return hipErrorInvalidValue;
+    const int k = res_nodes_per_thread(max_n);
+    if (k <= 1) launch_res<1>(args, table, ncav, parity, steps, s);
+    else if (k <= 2) launch_res<2>(args, table, ncav, parity, steps, s);
+    else if (k <= 4) launch_res<4>(args, table, ncav, parity, steps, s);
+    else if (k <= 6) launch_res<6>(args, table, ncav, parity, steps, s);
+    else if (k <= 8) launch_res<8>(args, table, ncav, parity, steps, s);
+    else if (k <= 9) launch_res<9>(args, table, ncav, parity, steps, s);
+    else launch_res<kResMaxK>(args, table, ncav, parity, steps, s);
     return hipGetLastError();
 }
 

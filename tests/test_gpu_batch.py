@@ -318,3 +318,128 @@ def test_batch_persistent_run_stops_like_single_cavities(hip):
     for x, y in zip(res[0], res[1]):
         assert np.array_equal(x, y)
     assert res[1][3].tolist() == [hip.LBM_OK, hip.LBM_ERR_NOT_CONVERGED, hip.LBM_OK]
+
+
+# cavities the resident variant takes (edges up to 70): odd and even edges,
+# a last node group of every size, the far wall on any lane
+RESIDENT = [dict(n=9, Ra=1e4, Pr=0.71, stretch=1.0),
+            dict(n=17, Ra=1e3, Pr=0.71, stretch=1.0),
+            dict(n=33, Ra=1e5, Pr=7.0, stretch=2.5),
+            dict(n=24, Ra=1e6, Pr=0.7, stretch=3.2),
+            dict(n=64, Ra=3e4, Pr=2.0, stretch=1.7),
+            dict(n=65, Ra=1e6, Pr=0.71, stretch=2.2),
+            dict(n=70, Ra=1e4, Pr=4.0, stretch=3.2)]
+
+
+def _resident_vs_single(hip, cav, chunks, launch_single=(64, 4)):
+    """Runs cav as a resident batch (variant 3) over the step counts in
+    chunks, and each cavity alone (variant 0) over the same chunks; returns
+    the largest relative difference of the fields per cavity and whether
+    all of them are equal to the bit."""
+    b = hip.Batch(cav)
+    b.set_launch(512, 1, variant=3)
+    assert b.get_launch()["variant"] == 3
+    b.init_rest()
+    for n in chunks:
+        b.step(n)
+    worst, exact = [], True
+    for i, c in enumerate(cav):
+        s = _single(hip, c)
+        s.set_launch(*launch_single, remap=0)
+        s.init_rest()
+        for n in chunks:
+            s.step(n)
+        assert b[i].itc == s.itc == sum(chunks)
+        got, ref = b[i].fields(), s.fields()
+        exact = exact and all(np.array_equal(x, y) for x, y in zip(got, ref))
+        worst.append(max(_rel(x, y) for x, y in zip(got, ref) if np.abs(y).max() > 0))
+        s.close()
+    b.close()
+    return worst, exact
+
+
+def test_resident_batch_matches_single_cavities(hip):
+    """Variant 3 keeps each cavity in one workgroup for all the steps of an
+    lbm_batch_step call.  Every step runs the single-cavity node code on the
+    same values, so each cavity agrees with its own variant-0 solver:
+    checked over one step, an odd and an even chunk (the final buffer
+    parity), and a chunk longer than one launch's 4096 steps."""
+    for chunks in ((1,), (137, 164), (4101,)):
+        print(f"\n[batch resident] {len(RESIDENT)} cavities, chunks {chunks}", flush=True)
+        worst, exact = _resident_vs_single(hip, RESIDENT, chunks)
+        print(f"  max rel diff per cavity {['%.2g' % w for w in worst]}, bit-exact {exact}",
+              flush=True)
+        assert max(worst) <= 1e-6, worst
+
+
+def test_resident_batch_matches_oracle(hip, oracle_mod):
+    print("\n[batch resident] vs oracle, 2 cavities, 1000 steps", flush=True)
+    cav = [RESIDENT[2], RESIDENT[5]]
+    b = hip.Batch(cav)
+    b.set_launch(512, 1, variant=3)
+    b.init_rest()
+    b.step(1000)
+    for i, c in enumerate(cav):
+        o = oracle_mod.Oracle(2, c["n"], c["Ra"], Pr=c["Pr"], A=c["stretch"])
+        o.init_rest()
+        o.step(1000)
+        errs = field_errors(b[i].fields(), o.fields())
+        print(f"  cavity {i} n={c['n']}: {errs}", flush=True)
+        for k in ("rho", "u", "v", "T"):
+            assert errs[k] <= 1e-5, (i, k, errs[k])
+        nh, nc = b[i].nusselt()
+        oh, oc = o.nusselt()
+        assert abs(nh - oh) <= 1e-4 * abs(oh) and abs(nc - oc) <= 1e-4 * abs(oc)
+        o.close()
+    b.close()
+
+
+def test_resident_batch_run_stops_like_single_cavities(hip):
+    """lbm_batch_run on variant 3: one resident launch per check window, each
+    cavity stopped by its own rule, with the statuses and iteration counts of
+    its own lbm_run and its residuals to FP32 rounding."""
+    print("\n[batch resident] run with per-cavity eps and itc_max", flush=True)
+    cav = [dict(n=17, Ra=1e3, epsU=1e-2, epsT=1e-2, itc_max=40000),
+           dict(n=24, Ra=1e4, epsU=1e-9, epsT=1e-9, itc_max=6000),     # stops at itc_max
+           dict(n=64, Ra=1e4, stretch=1.5, epsU=1e-3, epsT=1e-3, itc_max=40000)]
+    b = hip.Batch(cav, check_every=1000)
+    b.set_launch(512, 1, variant=3)
+    b.init_rest()
+    itc, eu, et, st = b.run()
+    print(f"  itc={itc.tolist()} status={st.tolist()} errU={eu.tolist()}", flush=True)
+    assert st.tolist() == [hip.LBM_OK, hip.LBM_ERR_NOT_CONVERGED, hip.LBM_OK]
+    assert b.active()[0] == 0
+    for i, c in enumerate(cav):
+        s = _single(hip, c, check_every=1000)
+        s.set_launch(64, 4, remap=0)
+        s.init_rest()
+        code, sitc, seu, set_ = s.run()
+        assert code == st[i] and sitc == itc[i], (i, code, sitc)
+        assert abs(seu - eu[i]) <= 1e-5 * seu and abs(set_ - et[i]) <= 1e-5 * set_
+        s.close()
+    b.close()
+
+
+def test_resident_variant_needs_small_cavities(hip):
+    """Variant 3 is refused when a cavity's edge is above 70, and the
+    autotuner then leaves it out; with small cavities it is a candidate and
+    the autotuner restores every cavity's state."""
+    b = hip.Batch([dict(n=33, Ra=1e4), dict(n=71, Ra=1e4)])
+    with pytest.raises(hip.LbmError) as e:
+        b.set_launch(512, 1, variant=3)
+    assert e.value.code == hip.LBM_ERR_ARG
+    b.init_rest()
+    b.autotune(3)
+    assert b.get_launch()["variant"] != 3
+    b.close()
+    b = hip.Batch(RESIDENT)
+    b.init_rest()
+    b.step(21)
+    before = [b[i].fields() for i in range(len(RESIDENT))]
+    b.autotune(3)
+    print(f"\n[batch resident] autotuned launch {b.get_launch()}", flush=True)
+    for i in range(len(RESIDENT)):
+        assert b[i].itc == 21
+        for x, y in zip(before[i], b[i].fields()):
+            assert np.array_equal(x, y)
+    b.close()
