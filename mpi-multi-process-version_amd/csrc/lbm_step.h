@@ -830,8 +830,12 @@ __device__ __forceinline__ void step_node(const StepArgs<R, S> &a, int i, int j,
                                            load_heat_shift<DIM, SHIFT>(a, idx));
 }
 
-template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, int WPE = 0, bool NT = false>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+// MAXT: the largest workgroup the kernel is launched with.  The register
+// budget per lane follows from it (1024 threads: 128 VGPRs; 256 threads:
+// all 512), and the FP64 3D node update needs more than 128.
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, int WPE = 0, bool NT = false,
+          int MAXT = 1024>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_step(StepArgs<R, S> a) {
     int tx, ty, tz;
     tile_coords((int)blockIdx.x, a.ntiles, a.gx, a.gy, a.remap, tx, ty, tz);

@@ -6,20 +6,27 @@
 
 namespace lbm {
 
-template <int DIM, bool NT>
+template <int DIM, bool NT, int MAXT>
 static void launch_k(bool macro, dim3 grid, dim3 block, const StepArgs<double> &a, hipStream_t s) {
     if (macro)
-        hipLaunchKernelGGL((k_step<DIM, double, double, false, true, 0, NT>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((k_step<DIM, double, double, false, true, 0, NT, MAXT>), grid, block, 0,
+                           s, a);
     else
-        hipLaunchKernelGGL((k_step<DIM, double, double, false, false, 0, NT>), grid, block, 0, s,
-                           a);
+        hipLaunchKernelGGL((k_step<DIM, double, double, false, false, 0, NT, MAXT>), grid, block,
+                           0, s, a);
 }
 
+// 3D blocks of up to 256 threads run the instance compiled for 256, whose
+// register budget holds the FP64 D3Q19 + D3Q7 node update without spilling
+// (the 1024-thread instance is capped at 128 VGPRs and spills about 70).
 template <int DIM>
 static void launch_k(bool macro, bool nt, dim3 grid, dim3 block, const StepArgs<double> &a,
                      hipStream_t s) {
-    if (nt) launch_k<DIM, true>(macro, grid, block, a, s);
-    else launch_k<DIM, false>(macro, grid, block, a, s);
+    const bool small = DIM == 3 && block.x * block.y <= 256;
+    if (nt && small) launch_k<DIM, true, 256>(macro, grid, block, a, s);
+    else if (nt) launch_k<DIM, true, 1024>(macro, grid, block, a, s);
+    else if (small) launch_k<DIM, false, 256>(macro, grid, block, a, s);
+    else launch_k<DIM, false, 1024>(macro, grid, block, a, s);
 }
 
 hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a0, const LaunchShape &ls,

@@ -361,15 +361,15 @@ def _resident_vs_single(hip, cav, chunks, launch_single=(64, 4)):
 def test_resident_batch_matches_single_cavities(hip):
     """Variant 3 keeps each cavity in one workgroup for all the steps of an
     lbm_batch_step call.  Every step runs the single-cavity node code on the
-    same values, so each cavity agrees with its own variant-0 solver:
-    checked over one step, an odd and an even chunk (the final buffer
-    parity), and a chunk longer than one launch's 4096 steps."""
+    same values in the same order, so each cavity equals its own variant-0
+    solver to the bit: checked over one step, an odd and an even chunk (the
+    final buffer parity), and a chunk longer than one launch's 4096 steps."""
     for chunks in ((1,), (137, 164), (4101,)):
         print(f"\n[batch resident] {len(RESIDENT)} cavities, chunks {chunks}", flush=True)
         worst, exact = _resident_vs_single(hip, RESIDENT, chunks)
         print(f"  max rel diff per cavity {['%.2g' % w for w in worst]}, bit-exact {exact}",
               flush=True)
-        assert max(worst) <= 1e-6, worst
+        assert exact, worst
 
 
 def test_resident_batch_matches_oracle(hip, oracle_mod):
