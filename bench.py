@@ -35,6 +35,9 @@ WORKLOADS = {
                   label="cavity3d_257_ra1e5_d3q19_d3q7"),
     "2d1025": dict(dim=2, n=1025, Ra=1e6, check_steps=20000,
                    label="cavity2d_1025_ra1e6_d2q9_d2q5"),
+    # a 2D grid whose step (235 MB of populations written) exceeds the MALL
+    "2d2049": dict(dim=2, n=2049, Ra=1e6, check_steps=2000,
+                   label="cavity2d_2049_ra1e6_d2q9_d2q5"),
     "batch64": dict(dim=2, count=64, seed=20261203,
                     label="batch_64_cavities2d_n65-513_A1-2.5_Ra1e3-1e6_Pr0.7-7"),
     # 256 seeded 64^2 cavities (the warm-start data generation's cavities),

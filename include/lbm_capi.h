@@ -211,10 +211,16 @@ int32_t lbm_copy_state(lbm_handle dst, lbm_handle src);
  *          column or row, that column or row is updated by a second,
  *          one-node-per-thread launch);
  *      2D: one wave per 64 columns, in y; bx = 64, by = 1.
+ *    2 (2D, FP32 storage, occupancy 0) = two time steps per launch: each
+ *    workgroup of 256 threads holds a 64 x (by + 4) region in registers and
+ *    LDS and writes its 60 x by output tile after two steps; bx = 64,
+ *    by = 8, 12 or 16 (an odd step count ends with a one-step launch).
+ *    Its results equal variant 0's to the bit.
  *    The variants read the same stencil values and agree to FP32 rounding.
  * Defaults set by lbm_create: 3D FP32 with 6 <= n <= 544: variant
- * 1, 64 x 4, remap 2; 2D with n <= 384: 32 x 2, remap 1; otherwise 64 x 4,
- * remap 1, variant 0. */
+ * 1, 64 x 4, remap 2; 2D with n <= 384: 32 x 2, remap 1; 2D FP32 with
+ * n > 384: variant 2, 64 x 8, remap 1; otherwise 64 x 4, remap 1,
+ * variant 0. */
 typedef struct lbm_launch {
     int32_t bx, by, remap, occupancy, variant;
 } lbm_launch;
@@ -300,7 +306,11 @@ int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms);
  *  3: resident cavities (bx = 512, by = 1; every cavity's edge at most 70):
  *     one workgroup per cavity holds the cavity's populations in registers
  *     and LDS and runs all the steps of an lbm_batch_step call (up to 4096
- *     per launch) in one launch; results equal variant 0's to the bit. */
+ *     per launch) in one launch; results equal variant 0's to the bit;
+ *  4: two steps per launch (bx = 64, by = 8, 12 or 16): tiles of 60 x by
+ *     nodes of every cavity, each held with a two-node halo in registers
+ *     and LDS for two steps (the single-cavity variant 2); results equal
+ *     variant 0's to the bit. */
 int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *launch);
 int32_t lbm_batch_get_launch(lbm_batch b, lbm_launch *launch);
 int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms);

@@ -46,7 +46,8 @@ struct lbm_batch_ctx {
     // 132.4)
     int bx = 64, by = 8;
     int variant = 0;   // 0: one node per thread; 1: y-march strips; 2: persistent;
-                       // 3: resident (one workgroup per cavity)
+                       // 3: resident (one workgroup per cavity); 4: two steps
+                       // per launch on 60 x by tiles
     int kc = 0;        // variant 1: rows per chunk
     // variant 2 (one persistent launch per lbm_batch_step call): per-cavity
     // (first table entry, gx, gy), the MALL group offsets, the ticket
@@ -280,7 +281,8 @@ int32_t rebuild(lbm_batch b) {
                 gnodes = 0.0;
             }
             gnodes += nodes;
-            const int gx = (h.n + b->bx - 1) / b->bx, gy = (h.n + b->by - 1) / b->by;
+            const int tw = b->variant == 4 ? lbm::t2_out_columns() : b->bx;
+            const int gx = (h.n + tw - 1) / tw, gy = (h.n + b->by - 1) / b->by;
             for (int ty = 0; ty < gy; ty++)
                 for (int tx = 0; tx < gx; tx++)
                     tiles.push_back(make_int4(c, tx, ty, 2 * c + h.cur));
@@ -343,6 +345,40 @@ int32_t batch_steps(lbm_batch b, int64_t nsteps) {
         for (size_t c = 0; c < b->cav.size(); c++) {
             if (!b->active[c]) continue;
             b->cav[c]->cur ^= (int)(nsteps & 1);
+            b->cav[c]->itc += nsteps;
+        }
+        return LBM_OK;
+    }
+    if (b->variant == 4) {
+        // passes of two steps (one launch and one buffer flip each), a
+        // one-step pass for an odd count; MALL groups run up to
+        // kGroupDepth steps each before the next group starts
+        int64_t flips = 0;
+        const int64_t depth = ng <= 1 ? nsteps : kGroupDepth;
+        for (int64_t s0 = 0; s0 < nsteps; s0 += depth) {
+            const int64_t d = std::min(depth, nsteps - s0);
+            int64_t launches = 0;
+            for (int g = 0; g < ng; g++) {
+                const int t0 = b->groups[g], cnt = b->groups[g + 1] - t0;
+                const bool nt = lbm::stream_stores(
+                    (ng <= 1 ? nodes : b->group_nodes[g]) * kStepBytesPerNode);
+                launches = 0;
+                for (int64_t s = 0; s < d;) {
+                    const int nstep = d - s >= 2 ? 2 : 1;
+                    hipError_t e = lbm::launch_step2_batch_f32(
+                        nstep, s0 + s + nstep == nsteps, b->d_args, b->d_tiles + t0, cnt,
+                        b->parity ^ (int)(launches & 1), b->by, nt, b->stream);
+                    if (e != hipSuccess) return hip_err(e, "two-step batch launch");
+                    s += nstep;
+                    launches++;
+                }
+            }
+            b->parity ^= (int)(launches & 1);
+            flips += launches;
+        }
+        for (size_t c = 0; c < b->cav.size(); c++) {
+            if (!b->active[c]) continue;
+            b->cav[c]->cur ^= (int)(flips & 1);
             b->cav[c]->itc += nsteps;
         }
         return LBM_OK;
@@ -657,11 +693,13 @@ int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms) {
 int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *l) {
     if (!b || !l) return set_error(LBM_ERR_ARG, "null argument");
     if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
-        l->remap != 0 || l->occupancy != 0 || l->variant < 0 || l->variant > 3 ||
+        l->remap != 0 || l->occupancy != 0 || l->variant < 0 || l->variant > 4 ||
         (l->variant == 1 && (l->bx != 64 || l->by != 1)) ||
-        (l->variant == 3 && (l->bx != 512 || l->by != 1)))
+        (l->variant == 3 && (l->bx != 512 || l->by != 1)) ||
+        (l->variant == 4 && !lbm::t2_shape_ok(l->bx, l->by)))
         return set_error(LBM_ERR_ARG, "invalid batched launch configuration (variant 1: bx = 64, "
-                                      "by = 1; variant 3: bx = 512, by = 1)");
+                                      "by = 1; variant 3: bx = 512, by = 1; variant 4: bx = 64, "
+                                      "by = 8, 12 or 16)");
     if (l->variant == 3 && !resident_fits(b))
         return set_error(LBM_ERR_ARG, "variant 3 (resident cavities) needs every cavity's edge "
                                       "to be at most " + std::to_string(lbm::res_max_edge()));
@@ -698,11 +736,13 @@ int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms
         if ((rc = lbm_save_state(b->cav[c], saved[c].data(), bytes))) return rc;
     }
     // (bx, by, variant); variant 1 is the y-march, variant 2 the
-    // persistent launch, variant 3 the resident cavities
+    // persistent launch, variant 3 the resident cavities, variant 4 two
+    // steps per launch
     static const int cand[][3] = {{64, 1, 0},  {64, 2, 0},  {64, 4, 0},  {64, 8, 0},
                                   {128, 1, 0}, {128, 2, 0}, {128, 4, 0}, {256, 1, 0},
                                   {256, 2, 0}, {32, 8, 0},  {64, 1, 1},  {64, 4, 2},
-                                  {64, 8, 2},  {128, 4, 2}, {32, 8, 2},  {512, 1, 3}};
+                                  {64, 8, 2},  {128, 4, 2}, {32, 8, 2},  {512, 1, 3},
+                                  {64, 8, 4},  {64, 12, 4}, {64, 16, 4}};
     const int bx0 = b->bx, by0 = b->by, v0 = b->variant;
     int best_bx = bx0, best_by = by0, best_v = v0;
     double best = 1e300;

@@ -231,7 +231,24 @@ int32_t check_handle(lbm_handle h, bool need_init) {
     return set_device(*h);
 }
 
+// Launch variant 2 of a 2D FP32 cavity: passes of two steps (each flips the
+// buffer parity once), then a one-step pass for an odd count; the last step
+// writes the macroscopic fields.
+int32_t launch_steps_t2(Ctx &c, int64_t nsteps) {
+    for (int64_t s = 0; s < nsteps;) {
+        const int nstep = nsteps - s >= 2 ? 2 : 1;
+        bind_buffers(c);
+        hipError_t e = lbm::launch_step2_f32(nstep, s + nstep == nsteps, c.a32, c.shape, c.stream);
+        if (e != hipSuccess) return hip_fail(e, "two-step launch");
+        c.cur = 1 - c.cur;
+        c.itc += nstep;
+        s += nstep;
+    }
+    return LBM_OK;
+}
+
 int32_t launch_steps(Ctx &c, int64_t nsteps) {
+    if (c.shape.variant == 2) return launch_steps_t2(c, nsteps);
     for (int64_t s = 0; s < nsteps; s++) {
         const bool macro = (s == nsteps - 1);
         bind_buffers(c);
@@ -277,6 +294,7 @@ int32_t do_steps(Ctx &c, int64_t nsteps) {
         HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
         const int32_t rc = launch_steps(c, nsteps);
         const hipError_t e = hipStreamEndCapture(c.stream, &graph);
+        c.gflip = c.cur ^ cur0;
         c.cur = cur0;
         c.itc = itc0;
         if (rc || e != hipSuccess) {
@@ -291,7 +309,7 @@ int32_t do_steps(Ctx &c, int64_t nsteps) {
         }
     }
     HIP_TRY(hipGraphLaunch(g, c.stream));
-    c.cur ^= (int)(nsteps & 1);
+    c.cur ^= c.gflip;
     c.itc += nsteps;
     return LBM_OK;
 }
@@ -372,6 +390,18 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
         c->shape.by = 2;
     }
     c->fp64 = p.precision == LBM_PREC_FP64;
+    // Larger 2D FP32 grids: two steps per launch on 60 x 8 tiles (variant
+    // 2).  Per step, with check intervals replayed as graphs: 513^2 7.21
+    // against 8.14 us, 1025^2 17.0 against 21.6 us; 2049^2 72 against
+    // 127 us (y-march) and 147 us (plain).  Up to 257^2 a pass has too few
+    // tiles, and one-node-per-thread steps are faster (129^2: 3.1 against
+    // 5.4 us).
+    if (p.dim == 2 && c->n > 384 && !c->fp64) {
+        c->shape.bx = 64;
+        c->shape.by = 8;
+        c->shape.remap = 1;
+        c->shape.variant = 2;
+    }
     // 3D FP32 population rows start on 128-byte lines (pitch a multiple of
     // 32 elements): the step kernels' row loads and streaming stores then
     // cover whole lines whatever n is
@@ -411,6 +441,12 @@ int32_t lbm_create(const lbm_params *params, lbm_handle *out) {
     const size_t pbytes = esz * (size_t)c->plane * (size_t)(c->qf + c->qg);
     ALLOC(c->pbuf[0], pbytes);
     ALLOC(c->pbuf[1], pbytes);
+    // the padding between population planes is never written by a step:
+    // zero it in both buffers so that a saved state does not depend on
+    // which buffer holds the current populations
+    for (void *b : c->pbuf)
+        if ((e = hipMemset(b, 0, pbytes)) != hipSuccess)
+            return cleanup(hip_fail(e, "population buffer clear"));
     const size_t fbytes = sizeof(double) * (size_t)c->nn;
     ALLOC(c->d_tref, sizeof(float) * (size_t)c->n);
     ALLOC(c->d_tcorr, sizeof(float) * 2 * (size_t)c->n);
@@ -786,8 +822,12 @@ int32_t lbm_set_launch(lbm_handle h, const lbm_launch *l) {
     if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
         l->remap < 0 || l->remap > 2 ||
         (l->occupancy != 0 && l->occupancy != 5 && l->occupancy != 6) ||
-        l->variant < 0 || l->variant > 1)
+        l->variant < 0 || l->variant > 2)
         return fail(LBM_ERR_ARG, "invalid launch configuration");
+    if (l->variant == 2 && (h->dim != 2 || h->fp64 || l->occupancy != 0 ||
+                            !lbm::t2_shape_ok(l->bx, l->by)))
+        return fail(LBM_ERR_ARG, "variant 2 (two steps per pass) needs a 2D cavity with FP32 "
+                                 "storage, bx = 64, by = 8, 12 or 16 and occupancy 0");
     if (l->variant == 1 && !zm_launch_ok(*h, l->bx, l->by, l->occupancy))
         return fail(LBM_ERR_ARG,
                     "variant 1 needs FP32 storage, bx = 64, occupancy 0 and, in 2D, by = 1; "
@@ -867,6 +907,23 @@ int32_t lbm_autotune(lbm_handle h, int32_t steps_per_trial, double *best_ms) {
                 best = h->shape;
             }
         }
+    }
+    // variant 2 (2D FP32): two steps per pass, tiles of 60 x 8, 12 or 16
+    if (h->dim == 2 && !h->fp64) {
+        for (int by : {8, 12, 16})
+            for (int remap = 0; remap < 2; remap++) {
+                h->shape = lbm::LaunchShape{64, by, remap, 0, 2};
+                double ms = 0;
+                if ((rc = do_steps(*h, 2)) || (rc = lbm_time_steps(h, steps_per_trial, &ms))) {
+                    release();
+                    return rc;
+                }
+                ms /= steps_per_trial;
+                if (ms < best_t) {
+                    best_t = ms;
+                    best = h->shape;
+                }
+            }
     }
     h->shape = best;
     // restore: the snapshot was taken from buffer cur0

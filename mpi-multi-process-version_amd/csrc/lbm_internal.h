@@ -18,7 +18,9 @@ struct LaunchShape {
     int bx = 64, by = 4;
     int remap = 1;
     int occ = 0;           // 3D FP32: target waves per SIMD (0, 5 or 6)
-    int variant = 0;       // 3D FP32: 1 = LDS-staged z-march (lbm_step_zm.h)
+    int variant = 0;       // FP32: 1 = LDS-staged z-march (3D, lbm_step_zm.h) or
+                           // y-march (2D, lbm_step_ym.h); 2 = two steps per
+                           // pass (2D, lbm_step_t2.h)
 };
 
 // Device-side state of one cavity.  Populations live in two ping-pong
@@ -67,6 +69,7 @@ struct Ctx {
     // from buffer p, captured for launch shape gshape.
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int64_t gsteps = 0;
+    int gflip = 0;              // buffer parity change of one replay
     LaunchShape gshape{};
 };
 
@@ -87,6 +90,20 @@ hipError_t launch_step_f32(int dim, bool macro, const StepArgs<float> &a,
                            const LaunchShape &ls, hipStream_t s);
 hipError_t launch_prologue_f32(int dim, const StepArgs<float> &a,
                                const StepArgs<double> &ad, hipStream_t s);
+// Launch variant 2 of a 2D FP32 cavity: one pass of nstep (2, or 1 for the
+// last step of an odd count) time steps (lbm_step_t2.h) from a.pin to
+// a.pout, tiles of 60 x ls.by output nodes; macro: the pass's last step
+// writes the FP64 fields (always, for a one-step pass).
+hipError_t launch_step2_f32(int nstep, bool macro, const StepArgs<float> &a, const LaunchShape &ls,
+                            hipStream_t s);
+bool t2_shape_ok(int bx, int by);
+// The batch's variant 4: one such pass over a table of (cavity, tile x,
+// tile y, argument index at parity 0) entries, tiles of t2_out_columns() x
+// ty output nodes; workgroup b uses args[tiles[b].w ^ parity].
+hipError_t launch_step2_batch_f32(int nstep, bool macro, const StepArgs<float> *args,
+                                  const int4 *tiles, int ntiles, int parity, int ty, bool nt,
+                                  hipStream_t s);
+int t2_out_columns();
 hipError_t launch_step_f64(int dim, bool macro, const StepArgs<double> &a,
                            const LaunchShape &ls, hipStream_t s);
 hipError_t launch_prologue_f64(int dim, const StepArgs<double> &a,

@@ -2,6 +2,7 @@
 // step (the production path).  Built with default FMA contraction.
 #include "lbm_internal.h"
 #include "lbm_step_res.h"
+#include "lbm_step_t2.h"
 #include "lbm_step_ym.h"
 #include "lbm_step_zm.h"
 
@@ -122,6 +123,69 @@ hipError_t launch_step_zm(bool macro, const StepArgs<float> &a0, const LaunchSha
     else if (ls.by == 6) launch_zm<6>(macro, nt, grid, block, a, kc, xz, yz, rim_wgs, s);
     else if (ls.by == 8) launch_zm<8>(macro, nt, grid, block, a, kc, xz, yz, rim_wgs, s);
     else launch_zm<4>(macro, nt, grid, block, a, kc, xz, yz, rim_wgs, s);
+    return hipGetLastError();
+}
+
+// Launch variant 2 (2D): two steps per pass on tiles of 60 x TY output
+// nodes (TY = 8, 12 or 16), k_step_t2 / k_step_t2_batch; a one-step pass
+// (the last step of an odd count) always writes the macroscopic fields.
+bool t2_shape_ok(int bx, int by) { return bx == kT2Width && (by == 8 || by == 12 || by == 16); }
+
+int t2_out_columns() { return kT2Out; }
+
+template <int TY, int NSTEP, bool MACRO, bool NT>
+static void launch_t2_kernel(dim3 grid, const StepArgs<float> *args, const int4 *tiles,
+                             int parity, const StepArgs<float> &a, hipStream_t s) {
+    const dim3 block(kT2Threads);
+    if (tiles)
+        hipLaunchKernelGGL((k_step_t2_batch<TY, NSTEP, MACRO, NT>), grid, block, 0, s, args,
+                           tiles, parity);
+    else
+        hipLaunchKernelGGL((k_step_t2<TY, NSTEP, MACRO, NT>), grid, block, 0, s, a);
+}
+
+template <int TY>
+static void launch_t2(int nstep, bool macro, bool nt, dim3 grid, const StepArgs<float> *args,
+                      const int4 *tiles, int parity, const StepArgs<float> &a, hipStream_t s) {
+    if (nstep == 1) {
+        if (nt) launch_t2_kernel<TY, 1, true, true>(grid, args, tiles, parity, a, s);
+        else launch_t2_kernel<TY, 1, true, false>(grid, args, tiles, parity, a, s);
+    } else if (macro) {
+        if (nt) launch_t2_kernel<TY, 2, true, true>(grid, args, tiles, parity, a, s);
+        else launch_t2_kernel<TY, 2, true, false>(grid, args, tiles, parity, a, s);
+    } else {
+        if (nt) launch_t2_kernel<TY, 2, false, true>(grid, args, tiles, parity, a, s);
+        else launch_t2_kernel<TY, 2, false, false>(grid, args, tiles, parity, a, s);
+    }
+}
+
+static void launch_t2_ty(int ty, int nstep, bool macro, bool nt, dim3 grid,
+                         const StepArgs<float> *args, const int4 *tiles, int parity,
+                         const StepArgs<float> &a, hipStream_t s) {
+    if (ty == 8) launch_t2<8>(nstep, macro, nt, grid, args, tiles, parity, a, s);
+    else if (ty == 12) launch_t2<12>(nstep, macro, nt, grid, args, tiles, parity, a, s);
+    else launch_t2<16>(nstep, macro, nt, grid, args, tiles, parity, a, s);
+}
+
+hipError_t launch_step2_f32(int nstep, bool macro, const StepArgs<float> &a0,
+                            const LaunchShape &ls, hipStream_t s) {
+    if (!t2_shape_ok(ls.bx, ls.by) || nstep < 1 || nstep > 2) return hipErrorInvalidValue;
+    StepArgs<float> a = a0;
+    a.gx = (a.n + kT2Out - 1) / kT2Out;
+    a.gy = (a.n + ls.by - 1) / ls.by;
+    a.ntiles = a.gx * a.gy;
+    a.remap = ls.remap == 0 ? 0 : 1;
+    const bool nt = stream_stores(step_bytes_f32(2, a.n));
+    launch_t2_ty(ls.by, nstep, macro, nt, dim3(a.ntiles), nullptr, nullptr, 0, a, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_step2_batch_f32(int nstep, bool macro, const StepArgs<float> *args,
+                                  const int4 *tiles, int ntiles, int parity, int ty, bool nt,
+                                  hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    if (!t2_shape_ok(kT2Width, ty) || nstep < 1 || nstep > 2) return hipErrorInvalidValue;
+    launch_t2_ty(ty, nstep, macro, nt, dim3(ntiles), args, tiles, parity, StepArgs<float>{}, s);
     return hipGetLastError();
 }
 

@@ -443,3 +443,52 @@ def test_resident_variant_needs_small_cavities(hip):
         for x, y in zip(before[i], b[i].fields()):
             assert np.array_equal(x, y)
     b.close()
+
+
+@pytest.mark.parametrize("ty", [8, 16])
+def test_two_step_batch_matches_single_cavities_to_the_bit(hip, ty):
+    """Variant 4 runs two steps per launch on 60 x ty tiles of every cavity
+    (the single-cavity variant 2 over the batch's table): each cavity equals
+    its own variant-0 solver to the bit, over an even and an odd chunk."""
+    cav = CAVITIES + [dict(n=129, Ra=1e6, Pr=0.71, stretch=2.0), dict(n=121, Ra=1e5)]
+    print(f"\n[batch two-step] {len(cav)} cavities, tiles 60 x {ty}", flush=True)
+    b = hip.Batch(cav)
+    b.set_launch(64, ty, variant=4)
+    assert b.get_launch()["variant"] == 4
+    b.init_rest()
+    b.step(120)
+    b.step(37)
+    for i, c in enumerate(cav):
+        s = _single(hip, c)
+        s.set_launch(64, 4, remap=0)
+        s.init_rest()
+        s.step(157)
+        assert b[i].itc == s.itc == 157
+        for x, y in zip(b[i].fields(), s.fields()):
+            assert np.array_equal(x, y), (i, c)
+        assert np.array_equal(b[i].save_state(), s.save_state()), (i, c)
+        s.close()
+    b.close()
+
+
+def test_two_step_batch_with_mall_groups(hip):
+    """A batch above half the MALL runs variant 4 group by group, 64 steps
+    (32 launches) per group at a time: every cavity equals its variant-0
+    single-cavity run to the bit over full and partial chunks."""
+    cav = [dict(n=513 if i % 3 else 257, Ra=10 ** (3 + 3 * i / 13), Pr=0.7 + 0.4 * i,
+                stretch=1.0 + 0.1 * i) for i in range(14)]
+    print(f"\n[batch two-step, MALL groups] {len(cav)} cavities", flush=True)
+    b = hip.Batch(cav)
+    b.set_launch(64, 12, variant=4)
+    b.init_rest()
+    b.step(70)
+    b.step(7)
+    for i in (0, 1, 5, 13):
+        s = _single(hip, cav[i])
+        s.set_launch(64, 4, remap=0)
+        s.init_rest()
+        s.step(77)
+        for x, y in zip(b[i].fields(), s.fields()):
+            assert np.array_equal(x, y), i
+        s.close()
+    b.close()

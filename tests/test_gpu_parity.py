@@ -444,9 +444,55 @@ def test_lds_zmarch_variant_is_rejected_where_it_cannot_run(hip):
         s.close()
 
 
+@pytest.mark.parametrize("n", [5, 9, 33, 61, 62, 63, 121, 122, 129, 181, 257])
+def test_two_step_variant_equals_the_plain_kernel(hip, n):
+    """Variant 2 runs two steps per launch on 60 x TY tiles held in
+    registers and LDS (an odd step count ends with one plain launch).  Each
+    node update is k_step's arithmetic on the values k_step reads, so the
+    whole state (populations, fields, itc) must equal variant 0's to the
+    bit: every tile height, both tile orders, even and odd step counts.
+    n = 61, 121, 181 leave a last tile of one column (clamped to start at
+    n - 2, overlapping its neighbour); n = 62, 122 one of two columns; n = 5,
+    9 a single tile smaller than the region."""
+    print(f"\n[two-step] n={n}", flush=True)
+    ref = {}
+    for steps in (1, 2, 7, 20):
+        s = hip.Solver(2, n, 1e5, stretch=2.0)
+        s.set_launch(64, 4, remap=1, variant=0)
+        s.init_rest()
+        s.step(steps)
+        ref[steps] = (s.save_state(), s.fields())
+        s.close()
+    for ty in (8, 12, 16):
+        for remap in (0, 1):
+            for steps in (1, 2, 7, 20):
+                s = hip.Solver(2, n, 1e5, stretch=2.0)
+                s.set_launch(64, ty, remap=remap, variant=2)
+                assert s.get_launch()["variant"] == 2
+                s.init_rest()
+                s.step(steps)
+                state, fields = s.save_state(), s.fields()
+                s.close()
+                assert np.array_equal(state, ref[steps][0]), (ty, remap, steps)
+                for x, y in zip(fields, ref[steps][1]):
+                    assert np.array_equal(x, y), (ty, remap, steps)
+
+
+def test_two_step_variant_is_rejected_where_it_cannot_run(hip):
+    cases = [(dict(dim=3, n=32), (64, 8)),
+             (dict(dim=2, n=64, precision=hip.PREC_FP64), (64, 8)),
+             (dict(dim=2, n=64), (64, 4)),           # by 8, 12 or 16
+             (dict(dim=2, n=64), (128, 8))]          # bx = 64
+    for kw, (bx, by) in cases:
+        s = hip.Solver(kw["dim"], kw["n"], 1e5, precision=kw.get("precision", hip.PREC_FP32_SHIFTED))
+        with pytest.raises(hip.LbmError):
+            s.set_launch(bx, by, variant=2)
+        s.close()
+
+
 @pytest.mark.parametrize("dim,n,prec,launch", [
     (2, 33, "fp32", None), (2, 64, "fp64", None), (3, 16, "fp32", (64, 4, 0, 1)),
-    (2, 65, "fp32", (64, 1, 1, 1))])
+    (2, 65, "fp32", (64, 1, 1, 1)), (2, 129, "fp32", (64, 16, 1, 2))])
 def test_graph_replays_match_direct_launches(hip, dim, n, prec, launch):
     """Grids up to 2^20 nodes replay a check interval (check_every = 2000
     steps) as a HIP graph of the same launches; any other step count is
@@ -484,6 +530,7 @@ def test_graph_replays_match_direct_launches(hip, dim, n, prec, launch):
     (3, 16, "fp64", None, 150, 1e-11),
     (2, 65, "fp32", (64, 4, 1, 0), 400, FP32_TOL),
     (2, 66, "fp32", (64, 1, 1, 1), 400, FP32_TOL),
+    (2, 125, "fp32", (64, 12, 0, 2), 400, FP32_TOL),
     (3, 32, "fp32", (64, 4, 2, 1), 300, FP32_TOL),
     (3, 32, "fp32", (64, 4, 1, 0), 300, FP32_TOL),
 ])
@@ -518,7 +565,8 @@ def test_trajectory_keeps_the_cavity_symmetries(hip, dim, n, prec, launch, steps
     (3, 32, "fp64", dict(bx=64, by=4, remap=1, variant=0)),
     (2, 129, "fp32", dict(bx=32, by=2, remap=1, variant=0)),
     (2, 129, "fp64", dict(bx=32, by=2, remap=1, variant=0)),
-    (2, 513, "fp32", dict(bx=64, by=4, remap=1, variant=0)),
+    (2, 513, "fp32", dict(bx=64, by=8, remap=1, variant=2)),
+    (2, 513, "fp64", dict(bx=64, by=4, remap=1, variant=0)),
 ])
 def test_default_launch_shapes(hip, dim, n, prec, want):
     """lbm_create's default launch (the z-march for 3D FP32 where it
