@@ -48,10 +48,10 @@ WORKLOADS = {
                       label="cavity2d_129_ra1e4_to_steady_state"),
     "warm64": dict(dim=2, n=64, held_out=100, seed=7, weights="warm64.npz", itc_max=600000,
                    label="warm_start_64x64_cavities_Ra1e3-1e6_Pr0.7-7_A1-2.5"),
-    "warm257": dict(dim=2, n=257, held_out=100, seed=7, weights="warm257.npz", itc_max=2000000,
+    "warm257": dict(dim=2, n=257, held_out=100, seed=7, weights="warm257w28.npz", itc_max=2000000,
                     label="warm_start_257x257_cavities_Ra1e3-1e6_Pr0.7-7_A1-2.5"),
     "warm3d128": dict(dim=3, n=128, held_out=1, cube=dict(Ra=1e5, Pr=0.71, stretch=3.2),
-                      weights="warm257.npz", itc_max=1200000, eps=2e-6,
+                      weights="warm257w28.npz", itc_max=1200000, eps=2e-6,
                       label="warm_start_128^3_cube_Ra1e5_Pr0.71_A3.2_slice_extruded"),
 }
 
@@ -319,7 +319,8 @@ def run_warm(args, cfg):
     import numpy as np
     import lbmhip
     import surrogate
-    weights = os.path.join(os.path.dirname(surrogate.WEIGHTS), cfg["weights"])
+    wname = args.weights or cfg["weights"]
+    weights = os.path.join(os.path.dirname(surrogate.WEIGHTS), wname)
     net = surrogate.load_net(weights)
     net_grid, net_width, net_params = net.grid, net.width, net.param_count
     with np.load(weights, allow_pickle=False) as z:
@@ -354,7 +355,7 @@ def run_warm(args, cfg):
         "config": {"workload": cfg["label"], "held_out": len(cavs), "seed": cfg.get("seed"),
                    "eps": eps, "check_every": surrogate.CHECK_EVERY,
                    "itc_max": cfg["itc_max"],
-                   "network": {"weights": "data/" + cfg["weights"], "grid": net_grid,
+                   "network": {"weights": "data/" + wname, "grid": net_grid,
                                "width": net_width, "params": net_params,
                                "train_samples": record.get("train_samples"),
                                "epochs": record.get("epochs"),
@@ -433,6 +434,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="oracle steps for the CPU baseline (0: automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--weights", default="",
+                    help="warm-start workloads: network weights under data/ instead of the "
+                         "workload's own")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 GPU run that the FP32 fields and Nu are compared with")

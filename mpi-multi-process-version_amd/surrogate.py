@@ -123,6 +123,10 @@ def solve(cavities, eps=EPS, itc_max=600000, net=None, keep_fields=False, G=GRID
             b[i].init_fields(u, v, None, T)
     b[0].synchronize()
     t1 = time.perf_counter()
+    # the batch's launch (one step or two steps per launch, or resident
+    # cavities for edges up to 70), chosen by timing; the results do not
+    # depend on it
+    b.autotune(2)
     itc, eu, et, st = b.run()
     t2 = time.perf_counter()
     out = dict(itc=itc, status=st, errU=eu, errT=et, nu=np.array(b.nusselt()[0]),

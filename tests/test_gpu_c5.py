@@ -1,5 +1,7 @@
 """Configuration C5 on its own workloads: the learned warm start (weights
-data/warm257.npz) on held-out seeded 257 x 257 cavities and on the 128^3
+data/warm257w28.npz: base width 28, 516 687 parameters, on a 128 x 128 erf
+grid, trained on 2028 steady 257 x 257 fields of the batched solver) on
+held-out seeded 257 x 257 cavities and on the 128^3
 cube (Ra 1e5, Pr 0.71, A 3.2) from the extruded 2D prediction of its
 cross-section, each run warm and cold to the same stopping rule
 (natural-convection_2dim_mono.f90:307-330: both residuals <= eps on
@@ -19,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def net257():
-    path = os.path.join(os.path.dirname(S.WEIGHTS), "warm257.npz")
+    path = os.path.join(os.path.dirname(S.WEIGHTS), "warm257w28.npz")
     net = S.load_net(path)
     yield net
     net.close()
