@@ -251,14 +251,15 @@ def run_steady(args, cfg):
 
     # the learned warm start on the same case, FP64 to the same rule
     import surrogate
-    weights = os.path.join(os.path.dirname(surrogate.WEIGHTS), "warm257.npz")
+    wname = args.weights or "warm257w28.npz"
+    weights = os.path.join(os.path.dirname(surrogate.WEIGHTS), wname)
     net = surrogate.load_net(weights)
     cav = dict(n=cfg["n"], Ra=cfg["Ra"], Pr=0.71, stretch=3.2, precision=lbmhip.PREC_FP64)
     rows, sm = surrogate.compare(net, [cav], eps=1e-8, itc_max=2000000)
     net.close()
     w = rows[0]
     out["warm_start"] = {
-        "weights": "data/warm257.npz", "precision": "f64", "eps": 1e-8,
+        "weights": "data/" + wname, "precision": "f64", "eps": 1e-8,
         "cold_itc": w["cold_itc"], "warm_itc": w["warm_itc"],
         "cold_s": round(w["cold_s"], 4), "warm_s": round(w["warm_s"], 4),
         "stop_wall_ratio": round(w["cold_s"] / w["warm_s"], 3),
