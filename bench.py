@@ -52,7 +52,7 @@ WORKLOADS = {
                     label="warm_start_257x257_cavities_Ra1e3-1e6_Pr0.7-7_A1-2.5"),
     "warm3d128": dict(dim=3, n=128, held_out=1, cube=dict(Ra=1e5, Pr=0.71, stretch=3.2),
                       weights="warm257w28.npz", itc_max=1200000, eps=2e-6,
-                      label="warm_start_128^3_cube_Ra1e5_Pr0.71_A3.2_slice_extruded"),
+                      label="warm_start_128^3_cube_Ra1e5_Pr0.71_A3.2_slice_extruded_64^3_presolve"),
 }
 
 

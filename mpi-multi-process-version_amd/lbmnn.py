@@ -198,6 +198,33 @@ def resample(y, s_src, s_dst, t_hot=1.0, t_cold=0.0):
     return np.stack([M @ aug[c] @ M.T for c in range(3)])
 
 
+def resample3(u, v, w, T, s_src, s_dst, t_hot=1.0, t_cold=0.0):
+    """3D cube fields [k][j][i] on the nodes s_src = x/H (every axis alike)
+    onto the nodes s_dst: piecewise-linear in each axis over the nodes plus
+    the six walls, whose values are the cube's boundary conditions (no-slip
+    everywhere, T = t_hot at y = 0 and t_cold at y = H, zero normal gradient
+    of T on the x and z walls).  Returns (u, v, w, T), each
+    [len(s_dst)]^3."""
+    n = len(s_src)
+    s = np.concatenate(([0.0], np.asarray(s_src, np.float64), [1.0]))
+    M = _lerp_matrix(s, s_dst)
+    out = []
+    for c, f in enumerate((u, v, w, T)):
+        aug = np.zeros((n + 2, n + 2, n + 2))
+        aug[1:-1, 1:-1, 1:-1] = np.asarray(f, np.float64).reshape(n, n, n)
+        if c == 3:
+            aug[1:-1, 0, 1:-1] = t_hot       # y = 0 (the j axis is the middle index)
+            aug[1:-1, -1, 1:-1] = t_cold
+            aug[:, :, 0] = aug[:, :, 1]      # x walls
+            aug[:, :, -1] = aug[:, :, -2]
+            aug[0] = aug[1]                  # z walls
+            aug[-1] = aug[-2]
+            aug[0, 0, :] = aug[-1, 0, :] = t_hot   # the edges of the heated walls
+            aug[0, -1, :] = aug[-1, -1, :] = t_cold
+        out.append(np.einsum("ka,jb,ic,abc->kji", M, M, M, aug, optimize=True))
+    return tuple(out)
+
+
 def extrude(u, v, T, n):
     """The 3D cube's fields from a 2D prediction on its (y, z) plane: the
     3D cavity has its heated walls at y = 0 and y = H and gravity along +z,

@@ -50,6 +50,10 @@ STRETCH = (1.0, 2.5)
 EPS = 1e-6
 CHECK_EVERY = 2000
 WEIGHTS = os.path.join(_HERE, "data", "warm64.npz")
+# A cube larger than this starts from a coarse cube of this edge, itself
+# started from the extruded prediction and run to COARSE_EPS (warm_fields).
+COARSE_3D = 64
+COARSE_EPS = 1e-5
 # Targets in data files: int16 multiples of 2^-13 (|y| < 4, rounding <= 6.1e-5,
 # far below the network's error).
 TARGET_STEP = 2.0 ** -13
@@ -237,12 +241,28 @@ def load_net(path=WEIGHTS, batch_max=32, device=0):
     return net
 
 
-def warm_fields(net, cavity, s):
+def warm_fields(net, cavity, s, coarse=COARSE_3D):
     """(u, v, w, T) for lbm_init_fields() of the Solver s from the network's
     prediction for cavity (Ra, Pr, stretch), on the nodes net_inputs()
     names and resampled onto the solver's grid where those differ; a 3D cube
     gets that plane extruded along x (lbmnn.extrude), the 2D cavity being
-    the cube's (y, z) cross-section."""
+    the cube's (y, z) cross-section.  A cube with more than `coarse` nodes
+    per edge gets, instead, the fields of a coarse cube of `coarse` nodes
+    (the same cavity, FP32, the z-march) started from the extruded
+    prediction and run to errU, errT <= COARSE_EPS (at most 400 000 steps),
+    resampled onto its nodes (lbmnn.resample3): the extruded plane has the
+    2D flow's structure, and the coarse run gives it the third dimension
+    (the no-slip x walls)."""
+    if s.dim == 3 and coarse and s.n > coarse:
+        c = lbmhip.Solver(3, coarse, cavity["Ra"], Pr=cavity["Pr"], stretch=cavity["stretch"],
+                          epsU=COARSE_EPS, epsT=COARSE_EPS, check_every=CHECK_EVERY,
+                          itc_max=400000)
+        c.init_fields(*warm_fields(net, dict(cavity, n=coarse), c, coarse=None))
+        c.run()
+        _, u, v, w, T = c.fields()
+        xc = c.grid() / c.info()["H"]
+        c.close()
+        return lbmnn.resample3(u, v, w, T, xc, s.grid() / s.info()["H"])
     info = s.info()
     x, nodes = net_inputs(cavity, s, net.grid)
     y = net.forward(x[None])[0]

@@ -54,7 +54,9 @@ def test_c5_held_out_257_cavities(net257):
 
 def test_c5_cube_128(net257):
     """The 128^3 cube at its rule eps = 2e-6 (its FP32 velocity residual
-    wanders between 1e-6 and 2e-6 for hundreds of thousands of steps)."""
+    wanders between 1e-6 and 2e-6 for hundreds of thousands of steps),
+    started from a 64^3 cube that the extruded prediction starts
+    (surrogate.warm_fields); the coarse run's time counts as warm time."""
     cav = dict(dim=3, n=128, Ra=1e5, Pr=0.71, stretch=3.2)
     print("\n[C5] 128^3 cube, warm vs cold to eps 2e-6", flush=True)
     rows, s = S.compare(net257, [cav], eps=2e-6, itc_max=1200000,
@@ -65,5 +67,6 @@ def test_c5_cube_128(net257):
     _report("128^3", s)
     assert s["judged"] == 1 and s["fallbacks"] == 0
     assert s["nu_rel_dev_max"] <= 1e-3
-    assert r["warm_band_itc"] <= r["cold_band_itc"]
+    assert r["warm_band_itc"] * 3 <= r["cold_band_itc"]
+    assert s["band_wall_ratio"] >= 3.0
     assert r["warm_itc"] <= r["cold_itc"]

@@ -97,6 +97,36 @@ def test_resample_keeps_the_source_nodes_and_the_wall_values():
     assert np.isclose(r[1, 0, 1], 0.5 * y[1, 0, 3])          # v: 0 on the wall y = 0
 
 
+def test_resample3_keeps_nodes_linear_fields_and_wall_values():
+    """The cube's resampling (the coarse-cube warm start): identity on the
+    source nodes, exact for T linear in y (the heated axis, the middle
+    index of [k][j][i]), and the wall values of the cube's boundary
+    conditions half a step outside the first node: no slip everywhere, the
+    hot wall at y = 0, zero normal gradient of T on the x and z walls."""
+    n = 9
+    xs, Hs = lbmnn.erf_nodes(n, 2.0)
+    ss = xs / Hs
+    rng = np.random.default_rng(4)
+    f = [rng.standard_normal((n, n, n)) for _ in range(4)]
+    for a, b in zip(lbmnn.resample3(*f, ss, ss), f):
+        assert np.allclose(a, b, rtol=0, atol=1e-14)
+    xt, Ht = lbmnn.erf_nodes(17, 1.4)
+    st = xt / Ht
+    K, J, I = np.meshgrid(ss, ss, ss, indexing="ij")
+    zero = np.zeros((n, n, n))
+    r = lbmnn.resample3(zero, zero, zero, 1.0 - J, ss, st)
+    Kt, Jt, It = np.meshgrid(st, st, st, indexing="ij")
+    assert np.allclose(r[3], 1.0 - Jt, rtol=0, atol=1e-14)
+    assert all(np.all(c == 0.0) for c in r[:3])
+    p = np.array([0.5 * ss[0], ss[4]])   # half-way to the wall, and a source node
+    u, v, w, T = lbmnn.resample3(*f, ss, p)
+    assert np.isclose(u[1, 1, 0], 0.5 * f[0][4, 4, 0])            # u = 0 on x = 0
+    assert np.isclose(w[0, 1, 1], 0.5 * f[2][0, 4, 4])            # w = 0 on z = 0
+    assert np.isclose(T[1, 0, 1], 0.5 * (1.0 + f[3][4, 0, 4]))    # T = 1 on y = 0
+    assert np.isclose(T[1, 1, 0], f[3][4, 4, 0])                  # dT/dx = 0 on x = 0
+    assert np.isclose(T[0, 1, 1], f[3][0, 4, 4])                  # dT/dz = 0 on z = 0
+
+
 def test_extrude_puts_the_plane_on_the_cubes_y_z_axes():
     n = 5
     u = np.arange(25.0).reshape(5, 5)
