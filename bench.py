@@ -561,7 +561,8 @@ def main():
                   "vs_fp64_gpu": vs_fp64},
     }
     if not args.no_cpu_baseline:
-        steps = args.cpu_steps or (10 if cfg["dim"] == 3 else 300)
+        # about 20-30 s of CPU work: 20 steps of 256^3, 300 of 1025^2
+        steps = args.cpu_steps or (20 if cfg["dim"] == 3 else 300)
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_threads, steps)
     else:
         out["cpu_baseline"] = None
