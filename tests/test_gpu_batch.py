@@ -492,3 +492,30 @@ def test_two_step_batch_with_mall_groups(hip):
             assert np.array_equal(x, y), i
         s.close()
     b.close()
+
+
+def test_two_step_batch_run_matches_one_step_launches(hip):
+    """lbm_batch_run on variant 4 (two steps per launch, check windows of
+    1000 steps and an itc_max that is not a multiple of two launches'
+    worth): the same statuses, iteration counts and residuals as variant
+    0, to the bit."""
+    print("\n[batch two-step] run with per-cavity eps and itc_max", flush=True)
+    cav = [dict(n=17, Ra=1e3, epsU=1e-2, epsT=1e-2, itc_max=40000),
+           dict(n=24, Ra=1e4, epsU=1e-9, epsT=1e-9, itc_max=6001),
+           dict(n=129, Ra=1e4, stretch=1.5, epsU=1e-3, epsT=1e-3, itc_max=40000)]
+    res = []
+    for bx, by, variant in ((64, 4, 0), (64, 8, 4)):
+        b = hip.Batch(cav, check_every=1000)
+        b.set_launch(bx, by, variant=variant)
+        b.init_rest()
+        res.append(b.run())
+        fields = [b[i].fields() for i in range(len(cav))]
+        b.close()
+        res[-1] = res[-1] + (fields,)
+    for x, y in zip(res[0][:4], res[1][:4]):
+        assert np.array_equal(x, y)
+    for fa, fb in zip(res[0][4], res[1][4]):
+        for x, y in zip(fa, fb):
+            assert np.array_equal(x, y)
+    assert res[1][0].tolist()[1] == 6001
+    assert res[1][3].tolist() == [hip.LBM_OK, hip.LBM_ERR_NOT_CONVERGED, hip.LBM_OK]

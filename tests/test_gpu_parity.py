@@ -478,6 +478,29 @@ def test_two_step_variant_equals_the_plain_kernel(hip, n):
                     assert np.array_equal(x, y), (ty, remap, steps)
 
 
+@pytest.mark.parametrize("n,steps,ty", [(65, 401, 8), (129, 400, 16)])
+def test_two_step_variant_matches_oracle(oracle_mod, hip, n, steps, ty):
+    """Variant 2 against the FP64 oracle directly, to the FP32 bar of the
+    plain path (an odd count ends with a one-step pass)."""
+    print(f"\n[two-step vs oracle] n={n} steps={steps} ty={ty}", flush=True)
+    s = hip.Solver(2, n, 1e5)
+    s.set_launch(64, ty, remap=1, variant=2)
+    o = oracle_mod.Oracle(2, n, 1e5)
+    s.init_rest()
+    o.init_rest()
+    s.step(steps)
+    o.step(steps)
+    errs = field_errors(s.fields(), o.fields())
+    print(f"  {errs}", flush=True)
+    for k in ("rho", "u", "v", "T"):
+        assert errs[k] <= 1e-5, (k, errs[k])
+    nh, nc = s.nusselt()
+    oh, oc = o.nusselt()
+    assert abs(nh - oh) <= 1e-4 * abs(oh) and abs(nc - oc) <= 1e-4 * abs(oc)
+    s.close()
+    o.close()
+
+
 def test_two_step_variant_is_rejected_where_it_cannot_run(hip):
     cases = [(dict(dim=3, n=32), (64, 8)),
              (dict(dim=2, n=64, precision=hip.PREC_FP64), (64, 8)),
