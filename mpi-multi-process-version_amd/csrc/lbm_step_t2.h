@@ -26,6 +26,11 @@
 //
 // The host flips the buffer parity once per pass; an odd step count ends
 // with a one-step pass.
+//
+// The LDS exchange of a step is written out here as in k_step_batch_res
+// rather than shared through one device function: with the step body in a
+// separate inlined function, the compiler gave the resident kernel 18 to 37
+// more VGPRs at 64^2 (and spills), so each kernel keeps its own copy.
 #pragma once
 
 #include "lbm_step.h"
