@@ -708,14 +708,15 @@ __device__ __forceinline__ void node_collide(const StepArgs<R, S> &a, uint32_t f
 
 // Steps 2-4 of a node once f(k), g(k) are interpolated: the thermal shift
 // correction, the wall rules (with the node's own P(k) in pcf, pcg), the
-// macroscopic fields with the lagged force, the collision and the stores.
-// sp_f, sp_g are the sums of the node's own P(k) (the zeroth moments that
-// carry rho(k-1), T(k-1) on the FP32 path).  node: the node's population
-// index; fnode: its field index.
+// macroscopic fields with the lagged force and the collision, whose result
+// P(k+1) is left in fp, gp.  sp_f, sp_g are the sums of the node's own P(k)
+// (the zeroth moments that carry rho(k-1), T(k-1) on the FP32 path).  fnode:
+// the node's field index.
 template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, bool NT = false>
-__device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node, uint32_t fnode,
-                                          const int (&idx)[3], const R *pcf, const R *pcg,
-                                          R *f, R *g, R sp_f, R sp_g, const HeatShift<R> &hs) {
+__device__ __forceinline__ void node_tail_collide(const StepArgs<R, S> &a, uint32_t fnode,
+                                                  const int (&idx)[3], const R *pcf,
+                                                  const R *pcg, R *f, R *g, R sp_f, R sp_g,
+                                                  const HeatShift<R> &hs, R *fp, R *gp) {
     constexpr bool EXACT = std::is_same<S, double>::value;
     using FL = Flow<DIM>;
     using HT = Heat<DIM>;
@@ -769,8 +770,33 @@ __device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node
     R F_prev;
     if constexpr (EXACT) F_prev = ldp(a.force, fnode);
     else F_prev = lagged_force<SHIFT>(a.gbeta, sp_f, sp_g, hs.tloc);
-    R fp[QF], gp[QG];
     node_collide<DIM, S, R, SHIFT, MACRO, NT>(a, fnode, f, g, F_prev, hs.tloc, fp, gp);
+}
+
+// The stores of a node's P(k+1); node: its population index.
+template <int DIM, typename S, bool NT, typename R>
+__device__ __forceinline__ void node_store(S *pout, long plane, uint32_t node, const R *fp,
+                                           const R *gp) {
+    constexpr int QF = Flow<DIM>::Q, QG = Heat<DIM>::Q;
+    static_for<0, QF>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        stp<NT>(pout + al * plane, node, S(fp[al]));
+    });
+    static_for<0, QG>([&](auto A) {
+        constexpr int al = decltype(A)::value;
+        stp<NT>(pout + (QF + al) * plane, node, S(gp[al]));
+    });
+}
+
+// node_tail_collide, then the stores.
+template <int DIM, typename S, typename R, bool SHIFT, bool MACRO, bool NT = false>
+__device__ __forceinline__ void node_tail(const StepArgs<R, S> &a, uint32_t node, uint32_t fnode,
+                                          const int (&idx)[3], const R *pcf, const R *pcg,
+                                          R *f, R *g, R sp_f, R sp_g, const HeatShift<R> &hs) {
+    constexpr int QF = Flow<DIM>::Q, QG = Heat<DIM>::Q;
+    R fp[QF], gp[QG];
+    node_tail_collide<DIM, S, R, SHIFT, MACRO, NT>(a, fnode, idx, pcf, pcg, f, g, sp_f, sp_g, hs,
+                                                   fp, gp);
     static_for<0, QF>([&](auto A) {
         constexpr int al = decltype(A)::value;
         stp<NT>(a.pout + al * a.plane, node, S(fp[al]));

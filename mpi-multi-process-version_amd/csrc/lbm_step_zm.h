@@ -35,6 +35,16 @@
 // computed once (a plan) and the loads and LDS writes of a plane are
 // straight-line code.
 //
+// The staging is double-buffered: plane k reads one buffer and writes the
+// rows of plane k+1 (and of the plane entering the z window) into the
+// other, once its node is collided and before its stores are issued; a wait
+// for every load of the plane follows, then the stores and one barrier.
+// The stores are issued inside a branch (live lanes only), and after such a
+// branch the compiler's count of outstanding memory operations no longer
+// tells the stores from the loads: a load consumed after the stores would
+// wait for the stores to complete too.  Consuming every load before the
+// stores leaves the stores in flight while the next plane computes.
+//
 // Any n from 6 up, as long as the 26 population planes stay within 32-bit
 // element offsets (checked by the host), FP32 storage, 3D, bx = 64,
 // by = BY.  The population rows are padded to a pitch of a multiple of 32
@@ -326,6 +336,30 @@ __device__ __forceinline__ void zm_ring_shift(float (&zr)[NZ][3], float (&zo)[NZ
 
 __device__ __forceinline__ int zm_window(int k, int n) { return min(max(k - 1, 0), n - 3); }
 
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt not waited on): every vector memory
+// operation issued so far has completed.
+__device__ __forceinline__ void zm_wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// The z stencil of plane k (the same for every lane), read through the
+// constant address space so that it is fetched with scalar loads: a vector
+// load here would be counted behind the previous plane's stores and wait for
+// them.  The tables are never written by a step.
+__device__ __forceinline__ AxisStencil<float> zm_stencil_z(const StepArgs<float> &a, int k) {
+    typedef const __attribute__((address_space(4))) int *coff_t;
+    typedef const __attribute__((address_space(4))) float *cwt_t;
+    const coff_t off = (coff_t)a.off + 2 * k;   // (o0, o2) pairs
+    const cwt_t wt = (cwt_t)a.wt + 6 * k;
+    AxisStencil<float> s;
+    s.o0 = off[0];
+    s.o2 = off[1];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        s.wp[b] = wt[b];
+        s.wm[b] = wt[3 + b];
+    }
+    return s;
+}
+
 // Tile (tx, ty) and z group tz of hardware workgroup P; false for the
 // workgroups past the last z group.  remap 2 (gy >= 8): workgroups are
 // dispatched round-robin over the 8 XCDs, and XCD x = P % 8 owns the tile
@@ -385,7 +419,9 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc, 
     using HT = Heat<3>;
     constexpr int QF = FL::Q, QG = HT::Q;
     constexpr int NZ = L::nz(), NZO = L::nzo();
-    __shared__ zm_f4 lds4[L::total() / 4];
+    // two staging buffers: plane k reads buffer (k - k0) & 1 and stages
+    // plane k + 1 (and the plane entering the z window) into the other
+    __shared__ zm_f4 lds4[2 * L::total() / 4];
     float *lds = reinterpret_cast<float *>(lds4);
     const int n = a.n;
     if ((int)blockIdx.x < rim_wgs) {
@@ -456,7 +492,10 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc, 
         constexpr int E2 = W == 0 ? 2 : (W < 0 ? 2 : 0);
         const int kn = k + 1;
         const bool more = kn < k1;
-        st[2] = load_stencil(a, k);
+        const int b = (k - k0) & 1;
+        const float *cur = lds + b * L::total();
+        float *nxt = lds + (b ^ 1) * L::total();
+        st[2] = zm_stencil_z(a, k);
         const uint32_t node = col + sz2 * k;
         const int idx[3] = {i, j, k};
         float pcf[QF], pcg[QG], f[QF], g[QG];
@@ -489,7 +528,7 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc, 
                 float v;
                 if constexpr (L::purez(p)) v = zr[L::zi(p)][E1];
                 else if constexpr (L::zpop(p)) v = zo[L::oi(p)][E1];
-                else v = zm_at<BY, p>(lds, l, r, 0, 0);
+                else v = zm_at<BY, p>(cur, l, r, 0, 0);
                 if constexpr (p < QF) pcf[p] = v;
                 else pcg[p - QF] = v;
             }
@@ -508,25 +547,30 @@ __global__ __launch_bounds__(64 * BY) void k_step_zm(StepArgs<float> a, int kc, 
                     const float *w = L::cz(p) > 0 ? st[2].wp : st[2].wm;
                     v = lerp3<false>(w, zr[z][E0], zr[z][E1], zr[z][E2]);
                 } else if constexpr (p < QF) {
-                    v = zm_interp_plane<BY, p>(lds, l, r, pcf[p], st);
+                    v = zm_interp_plane<BY, p>(cur, l, r, pcf[p], st);
                 } else {
-                    v = zm_interp_plane<BY, p>(lds, l, r, pcg[p - QF], st);
+                    v = zm_interp_plane<BY, p>(cur, l, r, pcg[p - QF], st);
                 }
                 if constexpr (p < QF) f[p] = v;
                 else g[p - QF] = v;
             }
         });
+        float fp[QF], gp[QG];
         if (live)
-            node_tail<3, float, float, true, MACRO, NT>(a, node, fcol + fsz * k, idx, pcf, pcg,
-                                                        f, g, sp_f, sp_g, hs);
-
-        lds_barrier();   // every read of this plane's staging is done
-        if (more) zm_commit<BY, false>(lds, pin_, np);
-        if constexpr (SL) zm_commit<BY, true>(lds, pz, nzs);
+            node_tail_collide<3, float, float, true, MACRO, NT>(a, fcol + fsz * k, idx, pcf, pcg,
+                                                                f, g, sp_f, sp_g, hs, fp, gp);
+        // The other buffer was last read before the previous plane's barrier
+        // (its in-plane rows by the previous plane's compute, its z rows by
+        // the ring_in of the plane before that), and is read next after this
+        // plane's barrier: one barrier per plane orders every LDS write.
+        if (more) zm_commit<BY, false>(nxt, pin_, np);
+        if constexpr (SL) zm_commit<BY, true>(nxt, pz, nzs);
+        zm_wait_loads();
+        if (live) node_store<3, float, NT>(a.pout, a.plane, node, fp, gp);
         lds_barrier();
         if constexpr (SL) {
             zm_ring_shift(zr, zo);
-            zm_ring_in<BY, 2>(lds, l, r, st, pzn, zr, zo);
+            zm_ring_in<BY, 2>(nxt, l, r, st, pzn, zr, zo);
         }
     };
 

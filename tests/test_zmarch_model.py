@@ -11,6 +11,9 @@ shapes the GPU tests use:
     and every global element loaded lies inside the population planes;
   * every value a lane reads (from LDS or from the ring) is the one the
     interpolation needs: the right population, plane, row and column;
+  * the barriers order every LDS access: no LDS element is written and
+    read between the same two barriers (where the waves of a workgroup run
+    in any order);
   * the tile map hands every (x tile, y tile, z group) to exactly one
     workgroup, and the march visits every plane of a group once, with the
     wall copies at k = 0 and k = n-1 and the ring shift exactly when the z
@@ -147,12 +150,18 @@ def tile_map(n, by, remap):
 KERNEL_ORDER = {0: (0, 1, 2), -1: (1, 0, 2), 1: (1, 2, 0)}
 
 
-def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
+def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER, buffers=2, barriers=1):
     """Replays one workgroup's march; returns the number of checked reads.
     order: ring entries read as stencil points (o0, node, o2) by the body
-    copy for k = 0 (-1), inside (0) and k = n-1 (+1).  Reads are checked for
-    the lanes at both tile edges, around the middle and around the last live
-    lane of a partial tile; the staging covers every lane."""
+    copy for k = 0 (-1), inside (0) and k = n-1 (+1).  buffers: staging
+    buffers (plane k reads buffer (k - k0) % buffers); barriers: per plane
+    (2: after the plane's reads and after its writes; 1: after its writes
+    only).  The kernel has two buffers and one barrier.
+    Reads are checked for the lanes at both tile edges, around the middle
+    and around the last live lane of a partial tile; the staging covers
+    every lane.  Every LDS access is tagged with the number of barriers
+    before it, and an element written and read under the same count is a
+    race between waves."""
     L = Layout(by)
     xz, yz = region(n, by)
     pitch = (n + 31) // 32 * 32                  # padded population row
@@ -161,13 +170,31 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
     k0, k1 = tz * kc, min(n, (tz + 1) * kc)
     assert xz - i0 >= 1 and yz - j0 >= 1
     lds = {}
+    epoch = 0
+    wrote, read, races = {}, {}, []   # (buffer, LDS index) -> last barrier count
     checked = 0
     # halo ownership: every halo value of a phase has a thread
     for z in (False, True):
         nh = sum(2 * L.H[p] for p in range(NP) if staged(p) and zpop(p) == z and L.hx[p])
         assert nh <= 64 * by
 
-    def stage(z, kk):
+    def barrier():
+        nonlocal epoch
+        epoch += 1
+
+    def write(b, idx, tag):
+        if read.get((b, idx)) == epoch:
+            races.append((b, idx, epoch))
+        lds[(b, idx)] = tag
+        wrote[(b, idx)] = epoch
+
+    def load(b, idx):
+        if wrote.get((b, idx)) == epoch:
+            races.append((b, idx, epoch))
+        read[(b, idx)] = epoch
+        return lds.get((b, idx))
+
+    def stage(z, kk, b):
         kk = min(kk, n - 1)
         rl, covered = L.rows[z], set()
         for w, l in itertools.product(range(by), range(64)):
@@ -185,7 +212,7 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
                 assert g_idx % 4 == 0 and ii + 3 < pitch     # aligned, inside the padded row
                 for e in range(4):
                     # values past the grid's edge are padding
-                    lds[base + e] = (p, kk, jj, ii + e) if ii + e < n else None
+                    write(b, base + e, (p, kk, jj, ii + e) if ii + e < n else None)
             h = l + 64 * w
             for p in range(NP):
                 if not (staged(p) and zpop(p) == z and L.hx[p]):
@@ -196,18 +223,18 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
                     jj, ih = j0 - L.hy[p] + rr, (i0 + 64 if side else i0 - 1)
                     if 0 <= jj < n and 0 <= ih < n:
                         idx = L.off[p] + rr * L.P[p] + (68 if side else 3)
-                        lds[idx] = (p, kk, jj, ih)
+                        write(b, idx, (p, kk, jj, ih))
                 h -= cnt
         assert covered == set(range(len(rl)))
 
-    def inplane(p, l, r, i, j, m):
+    def inplane(p, l, r, i, j, m, b):
         """The in-plane stage of z-population p at plane m: its LDS reads."""
         nonlocal checked
         cx, cy, _ = vel(p)
         ax_off = stencil(i, n) if cx else stencil(j, n)
         for d in (0,) + ax_off:
             dx, dy = (d, 0) if cx else (0, d)
-            got = lds.get(L.at(p, l, r, dx, dy))
+            got = load(b, L.at(p, l, r, dx, dy))
             assert got == (p, m, j + dy, i + dx), (p, m, i, j, dx, dy, got)
             checked += 1
 
@@ -216,7 +243,7 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
     lanes = [(l, r) for r in range(by) for l in sorted(c for c in cols if 0 <= c < 64)]
     ring = {}   # (lane, p) -> [tag of entry 0, 1, 2]
 
-    def ring_in(entry, m):
+    def ring_in(entry, m, b):
         for l, r in lanes:
             live = i0 + l < xz and j0 + r < yz
             i, j = min(i0 + l, xz - 1), min(j0 + r, yz - 1)
@@ -227,15 +254,18 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
                     tag = (p, min(m, n - 1))
                 else:
                     if live:
-                        inplane(p, l, r, i, j, m)
+                        inplane(p, l, r, i, j, m, b)
                     tag = (p, m)
                 ring.setdefault((l, r), {}).setdefault(p, [None] * 3)[entry] = tag
 
     ws = window(k0, n)
     for s in range(3):
-        stage(True, ws + s)
-        ring_in(s, ws + s)
-    stage(False, k0)
+        stage(True, ws + s, 0)
+        barrier()
+        ring_in(s, ws + s, 0)
+        barrier()
+    stage(False, k0, 0)
+    barrier()
 
     visited = []
 
@@ -246,6 +276,7 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
         more = k + 1 < k1
         assert SL == (more and window(k + 1, n) != window(k, n))
         E = order[W]
+        b = (k - k0) % buffers
         zo = stencil(k, n)
         want_planes = (k + zo[0], k, k + zo[1])
         for l, r in lanes:
@@ -264,17 +295,22 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
                 for dx, dy in itertools.product((0,) + (xo if cx else ()), (0,) + (yo if cy else ())):
                     idx = L.at(p, l, r, dx, dy)
                     if live:
-                        assert lds.get(idx) == (p, k, j + dy, i + dx), (p, k, i, j, dx, dy)
+                        assert load(b, idx) == (p, k, j + dy, i + dx), (p, k, i, j, dx, dy)
                         checked += 1
+        nb = (b + 1) % buffers
+        if barriers == 2:
+            barrier()   # every read of this plane's staging is done
         if more:
-            stage(False, k + 1)
+            stage(False, k + 1, nb)
         if SL:
-            stage(True, k + 2)
+            stage(True, k + 2, nb)
+        barrier()
+        if SL:
             for lr in ring:
                 for p in ring[lr]:
                     e = ring[lr][p]
                     ring[lr][p] = [e[1], e[2], None]
-            ring_in(2, k + 2)
+            ring_in(2, k + 2, nb)
 
     # the kernel's loop structure
     k = k0
@@ -291,6 +327,7 @@ def march(n, by, tx, ty, tz, kc, order=KERNEL_ORDER):
     if k == n - 1 and k < k1:
         plane(1, False, k)
     assert [v[2] for v in visited] == list(range(k0, k1))
+    assert not races, f"LDS race: {races[:3]}"
     return checked
 
 
@@ -308,7 +345,9 @@ SHAPES = [(12, 2, 0), (20, 4, 0), (68, 4, 1), (72, 2, 2), (96, 4, 0), (132, 4, 2
 @pytest.mark.parametrize("n,by,remap", SHAPES)
 def test_zmarch_indexing_model(n, by, remap):
     kc, gx, gy, nzc = tile_map(n, by, remap)
-    edge = lambda g: sorted({0, 1, g - 2, g - 1} & set(range(g)))
+    # the first and last tiles on each axis, and their neighbours on the
+    # smaller grids (the largest ones take minutes otherwise)
+    edge = lambda g: sorted(({0, 1, g - 2, g - 1} if n < 200 else {0, g - 1}) & set(range(g)))
     total = 0
     for tx, ty, tz in itertools.product(edge(gx), edge(gy), edge(nzc)):
         total += march(n, by, tx, ty, tz, kc)
@@ -326,6 +365,17 @@ def test_zmarch_model_catches_a_wrong_ring_order():
     with pytest.raises(AssertionError):
         march(n, by, 0, 0, nzc - 1, kc, order=flat)
     assert march(n, by, 0, 0, 0, kc) > 0
+
+
+@pytest.mark.parametrize("n,by", [(12, 2), (72, 4)])
+def test_zmarch_model_catches_a_missing_barrier(n, by):
+    """With one staging buffer, one barrier per plane lets the next plane's
+    rows land while slower waves still read the current ones: caught.  One
+    buffer needs a second barrier, after the plane's reads."""
+    kc, gx, gy, nzc = tile_map(n, by, 0)
+    with pytest.raises(AssertionError, match="LDS race"):
+        march(n, by, 0, 0, 0, kc, buffers=1, barriers=1)
+    assert march(n, by, 0, 0, 0, kc, buffers=1, barriers=2) > 0
 
 
 def test_tile_map_covers_every_tile_once_on_the_bench_grid():
