@@ -102,6 +102,16 @@ def cpu_baseline(cfg, threads, sample_steps):
                       f"(FP64 CPU oracle, {threads} OpenMP threads, {dt:.1f} s)"}
 
 
+BATCH_LAUNCHES = {
+    0: "single GPU, one launch per step for all cavities",
+    1: "single GPU, one launch per step for all cavities (y-march strips)",
+    3: "single GPU, one workgroup per cavity, up to 4096 steps per launch",
+    4: "single GPU, one launch per two steps for all cavities",
+    5: "single GPU, one cooperative launch per call (up to 4096 steps) for all cavities, "
+       "grid-wide barrier between steps",
+}
+
+
 def run_batch(args, cfg):
     """The batched sweep: every cavity advanced by one launch per step."""
     import numpy as np
@@ -148,9 +158,7 @@ def run_batch(args, cfg):
         "data": "synthetic: seeded cavity parameters, reference initial condition",
         "config": {"workload": cfg["label"], "cavities": len(cavs), "nodes": nodes,
                    "edges": sizes, "seed": cfg["seed"], "launch": launch,
-                   "parallelism": "single GPU, one workgroup per cavity, up to 4096 steps "
-                                  "per launch" if launch["variant"] == 3 else
-                                  "single GPU, one launch per step for all cavities"},
+                   "parallelism": BATCH_LAUNCHES.get(launch["variant"], BATCH_LAUNCHES[0])},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "algorithmic_bytes_per_launch": bpu * nodes},

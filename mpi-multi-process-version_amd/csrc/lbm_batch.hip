@@ -47,7 +47,8 @@ struct lbm_batch_ctx {
     int bx = 64, by = 8;
     int variant = 0;   // 0: one node per thread; 1: y-march strips; 2: persistent;
                        // 3: resident (one workgroup per cavity); 4: two steps
-                       // per launch on 60 x by tiles
+                       // per launch on 60 x by tiles; 5: cooperative (one
+                       // launch per call, grid barriers between steps)
     int kc = 0;        // variant 1: rows per chunk
     // variant 2 (one persistent launch per lbm_batch_step call): per-cavity
     // (first table entry, gx, gy), the MALL group offsets, the ticket
@@ -59,6 +60,9 @@ struct lbm_batch_ctx {
     int *h_ctl = nullptr;   // pinned: the abort flag, read at synchronising calls
     int nwg = 0, nwg_bx = 0, nwg_by = 0;
     uint64_t spin_ticks = 0;   // 2 s of the device's wall clock
+    // variant 5: resident workgroups of the cooperative kernel for the
+    // block shape nwg_coop_bx x nwg_coop_by (and store mode)
+    int nwg_coop = 0, nwg_coop_bx = 0, nwg_coop_by = 0, nwg_coop_nt = -1;
     double *h_red = nullptr;   // pinned host memory: 4 residual sums per cavity
     // variant 0: tile-table offsets of the MALL groups (whole cavities in
     // table order), groups[g] .. groups[g + 1]; one group when all fit
@@ -77,6 +81,11 @@ namespace {
 // not established, so lbm_batch_set_launch refuses the variant and the
 // autotuner skips it.
 constexpr bool kPersistentEnabled = false;
+
+// Variant 5 runs at most this many steps per cooperative launch (a
+// multiple of kGroupDepth, so that MALL-group chunks never straddle two
+// launches except at the end of a call).
+constexpr int64_t kCoopStepsPerLaunch = 4096;
 
 // Variant 3 runs at most this many steps per launch (a launch then lasts
 // tens of milliseconds even for the largest resident cavities).
@@ -206,6 +215,47 @@ int32_t persist_tables(lbm_batch b, const std::vector<int4> &tiles) {
     return LBM_OK;
 }
 
+// Variant 5's device copy of the MALL group offsets of the tile table just
+// built.
+int32_t coop_tables(lbm_batch b) {
+    if ((int)b->groups.size() > b->groups_cap) {
+        if (b->d_groups) BATCH_TRY(hipFree(b->d_groups));
+        b->d_groups = nullptr;
+        b->groups_cap = 0;
+        BATCH_TRY(hipMalloc(&b->d_groups, sizeof(int) * b->groups.size()));
+        b->groups_cap = (int)b->groups.size();
+    }
+    BATCH_TRY(hipMemcpy(b->d_groups, b->groups.data(), sizeof(int) * b->groups.size(),
+                        hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+
+// Workgroups of variant 5's cooperative launch: every workgroup of the
+// current block shape that the device holds at once (the launch is refused
+// beyond that), and no more than the largest MALL group has tiles.
+int32_t coop_workgroups(lbm_batch b, bool nt) {
+    if (b->nwg_coop <= 0 || b->nwg_coop_bx != b->bx || b->nwg_coop_by != b->by ||
+        b->nwg_coop_nt != (int)nt) {
+        int coop = 0;
+        BATCH_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, b->device));
+        if (!coop)
+            return set_error(LBM_ERR_HIP, "variant 5 needs cooperative launches, which this "
+                                          "device does not support");
+        int per_cu = 0, cus = 0;
+        BATCH_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, lbm::step_batch_coop_kernel(nt), b->bx * b->by, 0));
+        BATCH_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+        if (per_cu < 1)
+            return set_error(LBM_ERR_ARG, "variant 5: no workgroup of this block shape fits a "
+                                          "compute unit");
+        b->nwg_coop = per_cu * cus;
+        b->nwg_coop_bx = b->bx;
+        b->nwg_coop_by = b->by;
+        b->nwg_coop_nt = (int)nt;
+    }
+    return LBM_OK;
+}
+
 // Resident workgroups of the persistent kernel for the current block
 // shape, over all compute units.
 int32_t persist_workgroups(lbm_batch b, bool nt) {
@@ -311,6 +361,10 @@ int32_t rebuild(lbm_batch b) {
         int32_t rc = persist_tables(b, tiles);
         if (rc) return rc;
     }
+    if (b->variant == 5) {
+        int32_t rc = coop_tables(b);
+        if (rc) return rc;
+    }
     b->parity = 0;
     b->dirty = false;
     return LBM_OK;
@@ -400,6 +454,32 @@ int32_t batch_steps(lbm_batch b, int64_t nsteps) {
             hipError_t e = lbm::launch_step_batch_persist_f32(b->d_args, pl, b->bx, b->by, nwg,
                                                               nt, b->stream);
             if (e != hipSuccess) return hip_err(e, "persistent batch launch");
+            b->parity ^= (int)(d & 1);
+        }
+        for (size_t c = 0; c < b->cav.size(); c++) {
+            if (!b->active[c]) continue;
+            b->cav[c]->cur ^= (int)(nsteps & 1);
+            b->cav[c]->itc += nsteps;
+        }
+        return LBM_OK;
+    }
+    if (b->variant == 5) {
+        // one cooperative launch per kCoopStepsPerLaunch steps; MALL groups
+        // run kGroupDepth-step chunks inside it, as the per-step launches do
+        const bool nt = ng <= 1 && lbm::stream_stores(nodes * kStepBytesPerNode);
+        int32_t rc = coop_workgroups(b, nt);
+        if (rc) return rc;
+        int max_group = 0;
+        for (int g = 0; g < ng; g++) max_group = std::max(max_group, b->groups[g + 1] - b->groups[g]);
+        const int nwg = std::min(b->nwg_coop, max_group);
+        for (int64_t s0 = 0; s0 < nsteps; s0 += kCoopStepsPerLaunch) {
+            const int64_t d = std::min(kCoopStepsPerLaunch, nsteps - s0);
+            const lbm::CoopPlan pl{b->d_tiles, b->d_groups, ng, (int)d,
+                                   ng > 1 ? (int)std::min(kGroupDepth, d) : (int)d, b->parity,
+                                   s0 + d == nsteps ? 1 : 0};
+            hipError_t e = lbm::launch_step_batch_coop_f32(b->d_args, pl, b->bx, b->by, nwg, nt,
+                                                           b->stream);
+            if (e != hipSuccess) return hip_err(e, "cooperative batch launch");
             b->parity ^= (int)(d & 1);
         }
         for (size_t c = 0; c < b->cav.size(); c++) {
@@ -693,7 +773,7 @@ int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms) {
 int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *l) {
     if (!b || !l) return set_error(LBM_ERR_ARG, "null argument");
     if (l->bx < 32 || l->bx % 32 || l->by < 1 || l->bx * l->by > 1024 || (l->bx * l->by) % 64 ||
-        l->remap != 0 || l->occupancy != 0 || l->variant < 0 || l->variant > 4 ||
+        l->remap != 0 || l->occupancy != 0 || l->variant < 0 || l->variant > 5 ||
         (l->variant == 1 && (l->bx != 64 || l->by != 1)) ||
         (l->variant == 3 && (l->bx != 512 || l->by != 1)) ||
         (l->variant == 4 && !lbm::t2_shape_ok(l->bx, l->by)))
@@ -737,12 +817,13 @@ int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms
     }
     // (bx, by, variant); variant 1 is the y-march, variant 2 the
     // persistent launch, variant 3 the resident cavities, variant 4 two
-    // steps per launch
+    // steps per launch, variant 5 the cooperative launch
     static const int cand[][3] = {{64, 1, 0},  {64, 2, 0},  {64, 4, 0},  {64, 8, 0},
                                   {128, 1, 0}, {128, 2, 0}, {128, 4, 0}, {256, 1, 0},
                                   {256, 2, 0}, {32, 8, 0},  {64, 1, 1},  {64, 4, 2},
                                   {64, 8, 2},  {128, 4, 2}, {32, 8, 2},  {512, 1, 3},
-                                  {64, 8, 4},  {64, 12, 4}, {64, 16, 4}};
+                                  {64, 8, 4},  {64, 12, 4}, {64, 16, 4}, {64, 8, 5},
+                                  {128, 4, 5}, {64, 4, 5}};
     const int bx0 = b->bx, by0 = b->by, v0 = b->variant;
     int best_bx = bx0, best_by = by0, best_v = v0;
     double best = 1e300;

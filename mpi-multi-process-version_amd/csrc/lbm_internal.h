@@ -174,6 +174,26 @@ hipError_t launch_step_batch_persist_f32(const StepArgs<float> *args, const Pers
 // The persistent kernel, for occupancy queries.
 const void *step_batch_persist_kernel(bool nt);
 
+// The batch's variant 5: one cooperative launch of `steps` time steps over
+// the variant-0 tile table, cut into MALL groups groups[0..ngroups] and
+// chunks of `depth` steps (each group runs a chunk's steps before the next
+// group starts).  parity is the buffer parity of the launch's first step;
+// the launch's last step writes the macroscopic fields when macro_last.
+struct CoopPlan {
+    const int4 *tiles;
+    const int *groups;
+    int ngroups;
+    int steps, depth;
+    int parity;
+    int macro_last;
+};
+// Launches nwg workgroups of bx x by threads with hipLaunchCooperativeKernel
+// (which refuses a grid that is not resident at once).
+hipError_t launch_step_batch_coop_f32(const StepArgs<float> *args, const CoopPlan &pl, int bx,
+                                      int by, int nwg, bool nt, hipStream_t s);
+// The cooperative kernel, for occupancy queries.
+const void *step_batch_coop_kernel(bool nt);
+
 constexpr int kRedThreads = 256;
 constexpr int kRedMaxBlocks = 1024;
 

@@ -6,6 +6,8 @@
 #include "lbm_step_ym.h"
 #include "lbm_step_zm.h"
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
 #include <cmath>
 
@@ -433,6 +435,87 @@ hipError_t launch_step_batch_persist_f32(const StepArgs<float> *args, const Pers
     if (nt) hipLaunchKernelGGL(k_step_batch_persist<true>, grid, block, 0, s, args, pl);
     else hipLaunchKernelGGL(k_step_batch_persist<false>, grid, block, 0, s, args, pl);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// The batch's variant 5: one cooperative launch runs every step of an
+// lbm_batch_step call (up to the host's per-launch cap) over the tile table
+// of every cavity.  The grid holds only workgroups that are resident at
+// once (hipLaunchCooperativeKernel refuses a larger one).  Each workgroup
+// strides over the tiles of the current MALL group and updates them with
+// the single-cavity node code; the grid-wide barrier (cooperative groups'
+// grid sync: an agent-scope release and acquire around a device-memory
+// counter) orders every tile's step-s stores, across the XCDs' L2s, before
+// any step-(s + 1) load, which also covers the write-after-read hazard
+// (step s + 1 overwrites the buffer step s read).  Nothing waits on data,
+// so the launch ends once every workgroup has passed every barrier.
+//
+// Barriers: one after each step that the same group's next step follows.
+// When the launch moves on to the next MALL group, a group's next step
+// comes after the other groups' chunks, and with depth >= 2 each of those
+// holds a barrier of its own, so the move needs none; with depth 1 every
+// step ends with one.
+//
+// A launch that writes the macroscopic fields runs its last step in a tail
+// of its own, after one more barrier, rather than as a branch in the loop:
+// with both node updates in one loop body the compiler shares code between
+// them and fuses fewer products into FMAs, so the results would differ
+// from variant 0's in the last bits.  Separate, each update compiles as in
+// k_step_batch and the results equal variant 0's to the bit.
+template <bool MACRO, bool NT>
+__device__ __forceinline__ void coop_tiles(const StepArgs<float> *__restrict__ args,
+                                           const int4 *__restrict__ tiles, int t0, int t1, int p) {
+    for (int e = t0 + (int)blockIdx.x; e < t1; e += (int)gridDim.x) {
+        const int4 t = tiles[e];
+        const StepArgs<float> a = args[t.w ^ p];
+        const int i = t.y * (int)blockDim.x + (int)threadIdx.x;
+        const int j = t.z * (int)blockDim.y + (int)threadIdx.y;
+        if (i < a.n && j < a.n) step_node<2, float, float, true, MACRO, NT>(a, i, j, 0);
+    }
+}
+
+template <bool NT>
+__global__ __launch_bounds__(1024) void k_step_batch_coop(const StepArgs<float> *__restrict__ args,
+                                                          CoopPlan pl) {
+    namespace cg = cooperative_groups;
+    const cg::grid_group grid = cg::this_grid();
+    if (!grid.is_valid()) return;   // not a cooperative launch: no barrier to meet
+    // steps of the loop; the tail runs the last one when it writes macros
+    const int nloop = pl.macro_last ? pl.steps - 1 : pl.steps;
+    for (int c0 = 0; c0 < nloop; c0 += pl.depth) {
+        const int d = min(pl.depth, nloop - c0);
+        for (int g = 0; g < pl.ngroups; g++) {
+            for (int s = 0; s < d; s++) {
+                const int step = c0 + s;
+                coop_tiles<false, NT>(args, pl.tiles, pl.groups[g], pl.groups[g + 1],
+                                      pl.parity ^ (step & 1));
+                const bool last = step == nloop - 1 && g == pl.ngroups - 1;
+                const bool group_move = s == d - 1 && pl.ngroups > 1 && d >= 2;
+                if (!last && !group_move) grid.sync();
+            }
+        }
+    }
+    if (pl.macro_last) {
+        if (nloop > 0) grid.sync();
+        const int p = pl.parity ^ ((pl.steps - 1) & 1);
+        for (int g = 0; g < pl.ngroups; g++)
+            coop_tiles<true, NT>(args, pl.tiles, pl.groups[g], pl.groups[g + 1], p);
+    }
+}
+
+const void *step_batch_coop_kernel(bool nt) {
+    return nt ? (const void *)k_step_batch_coop<true> : (const void *)k_step_batch_coop<false>;
+}
+
+hipError_t launch_step_batch_coop_f32(const StepArgs<float> *args, const CoopPlan &pl, int bx,
+                                      int by, int nwg, bool nt, hipStream_t s) {
+    if (pl.steps <= 0 || pl.ngroups <= 0 || nwg <= 0) return hipSuccess;
+    if (pl.depth < 1) return hipErrorInvalidValue;
+    CoopPlan plan = pl;
+    const StepArgs<float> *a = args;
+    void *kargs[] = {(void *)&a, (void *)&plan};
+    return hipLaunchCooperativeKernel(step_batch_coop_kernel(nt), dim3(nwg), dim3(bx, by, 1),
+                                      kargs, 0, s);
 }
 
 hipError_t launch_prologue_f32(int dim, const StepArgs<float> &a,
