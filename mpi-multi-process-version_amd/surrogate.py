@@ -49,6 +49,12 @@ PR = (0.7, 7.0)
 STRETCH = (1.0, 2.5)
 EPS = 1e-6
 CHECK_EVERY = 2000
+# The Nusselt stopping rule (settle_estimate): the last SETTLE_CHECKS hot-wall
+# Nu values bound the drift still to come by SETTLE_TOL (relative), on
+# SETTLE_HOLD checks in a row.
+SETTLE_TOL = 1e-4
+SETTLE_CHECKS = 4
+SETTLE_HOLD = 2
 WEIGHTS = os.path.join(_HERE, "data", "warm64.npz")
 # A cube larger than this starts from a coarse cube of this edge, itself
 # started from the extruded prediction and run to COARSE_EPS (warm_fields).
@@ -274,14 +280,53 @@ def warm_fields(net, cavity, s, coarse=COARSE_3D):
     return lbmnn.extrude(u, v, T, s.n)
 
 
-def traced_run(cavity, net=None, eps=EPS, itc_max=600000, log=None):
+def settle_estimate(nus):
+    """Bound on the relative drift |Nu_inf - Nu_k| / |Nu_k| still to come,
+    from the last SETTLE_CHECKS values Nu_k of a run's checks (equally
+    spaced in steps), or inf.
+
+    Near a steady state the slowest mode dominates, and successive
+    differences d_i shrink by a ratio rho_i = d_(i+1) / d_i with
+    |rho_i| < 1.  With r the largest |rho_i| of the window, the remaining
+    drift is at most |d_last| * r / (1 - r) (a geometric tail).  A window
+    whose differences do not shrink (r >= 1: still in the transient, or at
+    a turning point of an oscillation) gives inf."""
+    v = np.asarray(nus[-SETTLE_CHECKS:], dtype=np.float64)
+    if v.size < SETTLE_CHECKS or not np.all(np.isfinite(v)) or v[-1] == 0.0:
+        return np.inf
+    a = np.abs(np.diff(v))
+    if a[-1] == 0.0 and np.all(a == 0.0):
+        return 0.0
+    if np.any(a[:-1] == 0.0):
+        return np.inf
+    r = float(np.max(a[1:] / a[:-1]))
+    if r >= 1.0:
+        return np.inf
+    return float(a[-1] * r / (1.0 - r) / abs(v[-1]))
+
+
+def settle_entry(trace, tol=SETTLE_TOL):
+    """(itc, wall, Nu) of the first check at which the Nusselt stopping rule
+    holds: settle_estimate <= tol on SETTLE_HOLD checks in a row.  The rule
+    reads only the checks up to the one it stops at, so a run stopped by it
+    ends there; None if it never holds."""
+    held = 0
+    for k in range(len(trace)):
+        held = held + 1 if settle_estimate(trace[:k + 1, 2]) <= tol else 0
+        if held >= SETTLE_HOLD:
+            return trace[k, 0], trace[k, 1], trace[k, 2]
+    return None
+
+
+def traced_run(cavity, net=None, eps=EPS, itc_max=600000, log=None, stop="residual"):
     """One cavity to the stopping rule (the loop of lbm_run(): check every
     CHECK_EVERY steps, stop when errU and errT are both <= eps), from rest
     or from the network's prediction (FP32 shifted storage unless the
     cavity names a `precision`), recording after every check the step
     count, the wall time since the start (inference and lbm_init_fields
     included) and the hot-wall Nusselt number; log (if given) receives a
-    line every 50 checks."""
+    line every 50 checks.  stop="nu" stops instead at the Nusselt rule
+    (settle_entry), or at the residual rule if that comes first."""
     c = dict(cavity)
     s = lbmhip.Solver(c.pop("dim", 2), c.pop("n"), c.pop("Ra"), Pr=c.pop("Pr"),
                       stretch=c.pop("stretch"), epsU=eps, epsT=eps, check_every=CHECK_EVERY,
@@ -310,6 +355,9 @@ def traced_run(cavity, net=None, eps=EPS, itc_max=600000, log=None):
         if eu <= eps and et <= eps:
             status = lbmhip.LBM_OK
             break
+        if stop == "nu" and settle_entry(np.array(trace[-(SETTLE_CHECKS + SETTLE_HOLD):])):
+            status = lbmhip.LBM_OK
+            break
     s.close()
     return dict(status=status, itc=trace[-1][0], wall=trace[-1][1], nu=trace[-1][2],
                 trace=np.array(trace))
@@ -335,7 +383,10 @@ def compare(net, cavities, eps=EPS, itc_max=600000, tol=1e-3, log=None):
     back to the cold start, and its time is charged in full (warm attempt
     plus cold run).  Two times are compared: to the stopping rule, and to
     the check from which on the hot-wall Nusselt number stays within tol
-    of the cold run's converged value."""
+    of the cold run's converged value.  A third time needs no hindsight:
+    the first check at which the Nusselt stopping rule (settle_entry) holds,
+    read from the same traces (the rule uses only earlier checks), with the
+    Nu there compared against the cold run's converged value."""
     net.forward(np.zeros((1, 4, net.grid, net.grid), np.float32))   # load the kernels once
     rows = []
     for i, cav in enumerate(cavities):
@@ -358,6 +409,15 @@ def compare(net, cavities, eps=EPS, itc_max=600000, tol=1e-3, log=None):
                        if not fallback else 0.0,
                        cold_band_itc=int(cb[0]), cold_band_s=cb[1],
                        warm_band_itc=int(wb[0]) if wb else None, warm_band_s=wb[1] if wb else None)
+            cs = settle_entry(cold["trace"])
+            ws = settle_entry(warm["trace"])
+            if fallback:
+                ws = None if cs is None else (warm["itc"] + cs[0], warm["wall"] + cs[1], cs[2])
+            for key, st in (("cold", cs), ("warm", ws)):
+                row[f"{key}_settle_itc"] = None if st is None else int(st[0])
+                row[f"{key}_settle_s"] = None if st is None else float(st[1])
+                row[f"{key}_settle_nu_dev"] = (None if st is None else
+                                               float(abs(st[2] / cold["nu"] - 1.0)))
         rows.append(row)
         if log:
             log(json.dumps(dict(stage="compare", cavity=i, **row)))
@@ -370,7 +430,10 @@ def summarise(rows):
     if not ok:
         return dict(cavities=len(rows), judged=0)
     banded = [r for r in ok if r["warm_band_s"] is not None]
+    settled = [r for r in ok if r.get("cold_settle_s") is not None
+               and r.get("warm_settle_s") is not None]
     tot = lambda k, rs: float(sum(r[k] for r in rs))
+    devs = [r[k] for r in settled for k in ("cold_settle_nu_dev", "warm_settle_nu_dev")]
     return dict(
         cavities=len(rows), judged=len(ok),
         fallbacks=sum(r["fallback"] for r in ok),
@@ -384,7 +447,17 @@ def summarise(rows):
         band_wall_ratio_median=float(np.median([r["cold_band_s"] / r["warm_band_s"]
                                                 for r in banded])) if banded else None,
         cold_s=tot("cold_s", ok), warm_s=tot("warm_s", ok),
-        cold_band_s=tot("cold_band_s", banded), warm_band_s=tot("warm_band_s", banded))
+        cold_band_s=tot("cold_band_s", banded), warm_band_s=tot("warm_band_s", banded),
+        settle_cavities=len(settled),
+        settle_wall_ratio=(tot("cold_settle_s", settled) / tot("warm_settle_s", settled)
+                           if settled else None),
+        settle_itc_ratio=(tot("cold_settle_itc", settled) / tot("warm_settle_itc", settled)
+                          if settled else None),
+        settle_wall_ratio_median=float(np.median([r["cold_settle_s"] / r["warm_settle_s"]
+                                                  for r in settled])) if settled else None,
+        settle_nu_dev_max=max(devs) if devs else None,
+        settle_nu_dev_over_1e3=sum(d > 1e-3 for d in devs),
+        cold_settle_s=tot("cold_settle_s", settled), warm_settle_s=tot("warm_settle_s", settled))
 
 
 def save_data(path, y, record):

@@ -1,6 +1,7 @@
 """Host logic of the learned warm start (surrogate.py, lbmnn.py) -- CPU only:
 seeded cavity sampling, the network's input encoding, the Nusselt-band
-rule and the aggregation of warm-versus-cold results."""
+rule, the Nusselt stopping rule and the aggregation of warm-versus-cold
+results."""
 import numpy as np
 
 import lbmhip
@@ -151,6 +152,45 @@ def test_band_entry_is_the_last_entry_into_the_band():
     assert S.band_entry(nan, 2.0, 1e-3) is None
 
 
+def test_settle_estimate_is_the_geometric_tail():
+    """For Nu_k = Nu_inf + A r^k with 0 < r < 1 the bound is the exact
+    remaining drift (|d| r / (1 - r) = A r^k); for an alternating approach
+    (r < 0) it bounds it from above, by (1 + |r|) / (1 - |r|)."""
+    for r in (0.8, 0.5, -0.6):
+        k = np.arange(12)
+        nu = 5.0 + 0.1 * r ** k
+        for j in range(S.SETTLE_CHECKS, 13):
+            est = S.settle_estimate(nu[:j])
+            exact = abs(nu[j - 1] - 5.0) / nu[j - 1]
+            factor = 1.0 if r > 0 else (1 + abs(r)) / (1 - abs(r))
+            assert np.isclose(est, factor * exact, rtol=1e-9), (r, j, est, exact)
+    assert S.settle_estimate([2.0] * S.SETTLE_CHECKS) == 0.0
+    # too few values, growing differences (a transient), non-finite values
+    assert S.settle_estimate([1.0, 1.1, 1.2]) == np.inf
+    assert S.settle_estimate([1.0, 1.01, 1.03, 1.07]) == np.inf
+    assert S.settle_estimate([1.0, 1.01, np.nan, 1.02]) == np.inf
+    # a turning point: the differences shrink, then grow again
+    assert S.settle_estimate([1.0, 1.2, 1.21, 1.2]) == np.inf
+
+
+def test_settle_entry_is_the_first_check_where_the_rule_holds_twice():
+    r, A = 0.7, 0.2
+    k = np.arange(1, 61)
+    nu = 4.0 - A * r ** k
+    trace = np.stack([k * 2000, k * 0.25, nu], axis=1)
+    est = np.array([S.settle_estimate(nu[:j]) for j in range(1, 61)])
+    ok = est <= S.SETTLE_TOL
+    first = next(j for j in range(1, 60) if ok[j - 1] and ok[j])
+    itc, wall, got = S.settle_entry(trace)
+    assert (itc, wall, got) == (trace[first, 0], trace[first, 1], trace[first, 2])
+    # the drift left at the stop is within the rule's bound
+    assert abs(got / 4.0 - 1.0) <= S.SETTLE_TOL * 1.0001
+    # one check before, it did not yet hold twice
+    assert S.settle_entry(trace[:first]) is None
+    # a run that never settles
+    assert S.settle_entry(np.stack([k, k, 1.0 + 0.01 * k], axis=1)) is None
+
+
 def _row(cs, ws, ci, wi, cb, wb, status=lbmhip.LBM_OK, fallback=False, dev=1e-5):
     return dict(cold_status=status, warm_status=lbmhip.LBM_OK, cold_s=cs, warm_s=ws,
                 cold_itc=ci, warm_itc=wi, cold_band_s=cb[1], cold_band_itc=cb[0],
@@ -168,7 +208,15 @@ def test_summary_counts_only_cavities_the_cold_run_judged():
     assert np.isclose(s["band_wall_ratio"], 4.0 / 1.0)
     assert np.isclose(s["band_itc_ratio"], 40000 / 8000)
     assert np.isclose(s["nu_rel_dev_max"], 3e-4)
+    assert s["settle_cavities"] == 0 and s["settle_wall_ratio"] is None
     assert S.summarise([dict(cold_status=lbmhip.LBM_ERR_DIVERGED)])["judged"] == 0
+    rows[0].update(cold_settle_s=2.0, warm_settle_s=0.4, cold_settle_itc=20000,
+                   warm_settle_itc=4000, cold_settle_nu_dev=2e-4, warm_settle_nu_dev=1.2e-3)
+    rows[1].update(cold_settle_s=None, warm_settle_s=0.5)
+    s = S.summarise(rows)
+    assert s["settle_cavities"] == 1
+    assert np.isclose(s["settle_wall_ratio"], 5.0) and np.isclose(s["settle_itc_ratio"], 5.0)
+    assert np.isclose(s["settle_nu_dev_max"], 1.2e-3) and s["settle_nu_dev_over_1e3"] == 1
 
 
 def test_data_files_round_trip(tmp_path):
