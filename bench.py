@@ -399,8 +399,16 @@ def run_warm(args, cfg):
         cavs = surrogate.sample_cavities(cfg["held_out"], cfg["seed"], n=n)
     t0 = time.perf_counter()
     eps = cfg.get("eps", surrogate.EPS)
+    traces = [] if args.save_traces else None
     rows, s = surrogate.compare(net, cavs, eps=eps, itc_max=cfg["itc_max"],
-                                log=lambda r: print(r, file=sys.stderr, flush=True))
+                                log=lambda r: print(r, file=sys.stderr, flush=True),
+                                traces=traces)
+    if traces is not None:
+        np.savez_compressed(args.save_traces,
+                            **{f"{k}{i}": t[j] for i, t in enumerate(traces)
+                               for j, k in enumerate(("cold", "warm"))},
+                            cold_nu=np.array([r["cold_nu"] for r in rows]),
+                            cold_status=np.array([r["cold_status"] for r in rows]))
     wall = time.perf_counter() - t0
     net.close()
     steps = sum(r["cold_itc"] + r["warm_itc"] for r in rows)
@@ -509,6 +517,8 @@ def main():
                     help="warm-start workloads: network weights under data/ instead of the "
                          "workload's own")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--save-traces", default="",
+                    help="warm-start workloads: write every run's (itc, wall, Nu) checks to this .npz")
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 GPU run that the FP32 fields and Nu are compared with")
     ap.add_argument("--launch", default="",

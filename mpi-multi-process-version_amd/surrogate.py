@@ -375,7 +375,7 @@ def band_entry(trace, nu_ref, tol=1e-3):
     return trace[k, 0], trace[k, 1]
 
 
-def compare(net, cavities, eps=EPS, itc_max=600000, tol=1e-3, log=None):
+def compare(net, cavities, eps=EPS, itc_max=600000, tol=1e-3, log=None, traces=None):
     """Cold and warm start of each cavity to the same stopping rule.
 
     The cold run is the judge: only cavities whose cold run reaches the
@@ -392,6 +392,8 @@ def compare(net, cavities, eps=EPS, itc_max=600000, tol=1e-3, log=None):
     for i, cav in enumerate(cavities):
         cold = traced_run(cav, None, eps, itc_max, log)
         warm = traced_run(cav, net, eps, itc_max, log)
+        if traces is not None:   # (itc, wall, Nu) of every check, cold and warm
+            traces.append((cold["trace"], warm["trace"]))
         row = dict(Ra=cav["Ra"], Pr=cav["Pr"], stretch=cav["stretch"],
                    cold_status=cold["status"], warm_status=warm["status"],
                    cold_itc=int(cold["itc"]), warm_itc=int(warm["itc"]),
