@@ -446,7 +446,7 @@ def run_warm(args, cfg):
                                         "band_wall_ratio_median", "stop_wall_ratio",
                                         "stop_itc_ratio", "stop_wall_ratio_median", "cold_s",
                                         "warm_s", "cold_band_s", "warm_band_s",
-                                        "settle_cavities", "settle_wall_ratio",
+                                        "settle_cavities", "settle_by_nu", "settle_wall_ratio",
                                         "settle_itc_ratio", "settle_wall_ratio_median",
                                         "settle_nu_dev_max", "settle_nu_dev_over_1e3",
                                         "cold_settle_s", "warm_settle_s")},

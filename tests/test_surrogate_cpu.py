@@ -154,39 +154,49 @@ def test_band_entry_is_the_last_entry_into_the_band():
 
 def test_settle_estimate_is_the_geometric_tail():
     """For Nu_k = Nu_inf + A r^k with 0 < r < 1 the bound is the exact
-    remaining drift (|d| r / (1 - r) = A r^k); for an alternating approach
-    (r < 0) it bounds it from above, by (1 + |r|) / (1 - |r|)."""
-    for r in (0.8, 0.5, -0.6):
-        k = np.arange(12)
-        nu = 5.0 + 0.1 * r ** k
-        for j in range(S.SETTLE_CHECKS, 13):
+    remaining drift: the samples SETTLE_SPACING checks apart shrink by
+    rho = r^SETTLE_SPACING, and |d| rho / (1 - rho) = A r^k."""
+    span = (S.SETTLE_CHECKS - 1) * S.SETTLE_SPACING + 1
+    for r, A in ((0.95, 0.1), (0.8, -0.3), (0.99, 0.02)):
+        k = np.arange(span + 10)
+        nu = 5.0 + A * r ** k
+        for j in range(span, span + 11):
             est = S.settle_estimate(nu[:j])
             exact = abs(nu[j - 1] - 5.0) / nu[j - 1]
-            factor = 1.0 if r > 0 else (1 + abs(r)) / (1 - abs(r))
-            assert np.isclose(est, factor * exact, rtol=1e-9), (r, j, est, exact)
-    assert S.settle_estimate([2.0] * S.SETTLE_CHECKS) == 0.0
-    # too few values, growing differences (a transient), non-finite values
-    assert S.settle_estimate([1.0, 1.1, 1.2]) == np.inf
-    assert S.settle_estimate([1.0, 1.01, 1.03, 1.07]) == np.inf
-    assert S.settle_estimate([1.0, 1.01, np.nan, 1.02]) == np.inf
-    # a turning point: the differences shrink, then grow again
-    assert S.settle_estimate([1.0, 1.2, 1.21, 1.2]) == np.inf
+            assert np.isclose(est, exact, rtol=1e-8), (r, j, est, exact)
+        # one check short of a full window
+        assert S.settle_estimate(nu[:span - 1]) == np.inf
+    assert S.settle_estimate([2.0] * span) == 0.0
+    # an alternating approach (r < 0), growing differences (a transient),
+    # ratios that disagree, non-finite values: no bound
+    alt = np.repeat(5.0 + 0.1 * (-0.6) ** np.arange(S.SETTLE_CHECKS), S.SETTLE_SPACING)
+    assert S.settle_estimate(alt) == np.inf
+    assert S.settle_estimate(1.0 + 0.001 * 1.1 ** np.arange(span)) == np.inf
+    w = 5.0 + 0.1 * 0.9 ** np.arange(span)
+    w[-1] = w[-1 - S.SETTLE_SPACING] - 0.1 * (w[-1 - 2 * S.SETTLE_SPACING] - w[-1 - S.SETTLE_SPACING])
+    assert S.settle_estimate(w) == np.inf
+    nan = 5.0 + 0.1 * 0.9 ** np.arange(span)
+    nan[-1 - S.SETTLE_SPACING] = np.nan
+    assert S.settle_estimate(nan) == np.inf
 
 
 def test_settle_entry_is_the_first_check_where_the_rule_holds_twice():
-    r, A = 0.7, 0.2
-    k = np.arange(1, 61)
+    r, A = 0.93, 0.2
+    k = np.arange(1, 241)
     nu = 4.0 - A * r ** k
     trace = np.stack([k * 2000, k * 0.25, nu], axis=1)
-    est = np.array([S.settle_estimate(nu[:j]) for j in range(1, 61)])
+    est = np.array([S.settle_estimate(nu[:j]) for j in range(1, 241)])
     ok = est <= S.SETTLE_TOL
-    first = next(j for j in range(1, 60) if ok[j - 1] and ok[j])
+    first = next(j for j in range(1, 240) if ok[j - 1] and ok[j])
     itc, wall, got = S.settle_entry(trace)
     assert (itc, wall, got) == (trace[first, 0], trace[first, 1], trace[first, 2])
     # the drift left at the stop is within the rule's bound
     assert abs(got / 4.0 - 1.0) <= S.SETTLE_TOL * 1.0001
     # one check before, it did not yet hold twice
     assert S.settle_entry(trace[:first]) is None
+    # the span traced_run keeps is enough to decide at the latest check
+    assert S.settle_entry(trace[first + 1 - S.SETTLE_SPAN:first + 1]) is not None
+    assert S.settle_entry(trace[first - S.SETTLE_SPAN:first]) is None
     # a run that never settles
     assert S.settle_entry(np.stack([k, k, 1.0 + 0.01 * k], axis=1)) is None
 
