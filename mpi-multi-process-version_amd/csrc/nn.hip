@@ -51,35 +51,51 @@ struct GemmArgs {
     int M, N, K, kchunk;
 };
 
+// Read in place of an element outside an operand: a masked-out load reads
+// zeros from here, so every load is issued and its value used as it comes
+// (a value selected after the load lets the compiler turn the load into a
+// branch of its own, with a wait before the join).
+__device__ __align__(16) uint32_t kZeros[4] = {0u, 0u, 0u, 0u};   // never written
+
 // The 16 x 16 x 16 MFMA's operand fragment of one lane: lane (r, q) holds
 // X(r, 4q + e), e = 0..3, of the 16-row slice starting at row `row` and the
 // k range [k0, k0 + 16) clipped to [.., ke), with X(i, k) = X[i si + k sk]
-// and rows >= rows zero.  When k is the contiguous axis (sk == 1) of an
-// 8-byte aligned operand and ke is a multiple of 4, the four values are
-// one 8-byte load.
+// and rows >= rows zero.  VEC: k is the contiguous axis (sk == 1) of an
+// 8-byte aligned operand and every k range ends on a multiple of 4, so the
+// four values are one 8-byte load.  Every load is issued (from kZeros for a
+// masked-out element): no branch separates the loads of a k step, so they
+// are in flight together.
+template <bool VEC>
 __device__ __forceinline__ v4s fragment(const bf16 *X, long si, long sk, long i, bool iok,
-                                        long k, long ke, bool vec) {
+                                        long k, long ke) {
     v4s f;
-    if (vec) {
-        uint2 w = make_uint2(0u, 0u);
-        if (iok && k < ke) w = *(const uint2 *)(X + i * si + k);
+    if constexpr (VEC) {
+        const bool ok = iok && k < ke;
+        const uint2 w = *(ok ? (const uint2 *)(X + i * si + k) : (const uint2 *)kZeros);
         f[0] = (short)(w.x & 0xffffu);
         f[1] = (short)(w.x >> 16);
         f[2] = (short)(w.y & 0xffffu);
         f[3] = (short)(w.y >> 16);
-        return f;
-    }
+    } else {
 #pragma unroll
-    for (int e = 0; e < 4; e++)
-        f[e] = (iok && k + e < ke) ? (short)X[i * si + (k + e) * sk] : (short)0;
+        for (int e = 0; e < 4; e++) {
+            const bool ok = iok && k + e < ke;
+            f[e] = (short)*(ok ? X + i * si + (k + e) * sk : (const bf16 *)kZeros);
+        }
+    }
     return f;
 }
 
 // One wave per MT*16 x 16 block of C: MT 16 x 16 MFMA tiles down the m axis
 // share each B fragment (B, the large operand, is then read once per MT*16
 // rows of C).  The host picks MT and the k split per product so that a
-// launch has enough waves.
-template <int MT> __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+// launch has enough waves, and AV / BV (the 8-byte fragment loads of A / B)
+// where the layouts allow them.  The fragments of step k0 + 16 are loaded
+// while the MFMAs of step k0 run (two register sets in turn); each tile's
+// MFMAs run in k order, and tiles past M multiply zero fragments and are
+// not stored.
+template <int MT, bool AV, bool BV>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int tn = (g.N + 15) >> 4, tg = (g.M + 16 * MT - 1) / (16 * MT);
@@ -90,23 +106,33 @@ template <int MT> __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     const long n = n0 + r;
     const bool nok = n < g.N;
     const int nt = min(MT, (g.M - mb + 15) >> 4);   // m tiles in this block (wave-uniform)
-    const bool avec = g.as_k == 1 && (g.as_m & 3) == 0 && ((uintptr_t)g.A & 7) == 0 &&
-                      (ke & 3) == 0;
-    const bool bvec = g.bs_k == 1 && (g.bs_n & 3) == 0 && ((uintptr_t)g.B & 7) == 0 &&
-                      (ke & 3) == 0;
     v4f acc[MT];
-#pragma unroll
-    for (int t = 0; t < MT; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        const long k = k0 + 4 * q;
-        const v4s b = fragment(g.B, g.bs_n, g.bs_k, n, nok, k, ke, bvec);
+    // two fragment sets in turn: the loads of one k step are in flight
+    // while the MFMAs of the other run
+    v4s a0[MT], a1[MT], b0, b1;
+    auto load = [&](long k, v4s (&a)[MT], v4s &b) {
+        b = fragment<BV>(g.B, g.bs_n, g.bs_k, n, nok, k, ke);
 #pragma unroll
         for (int t = 0; t < MT; t++) {
-            if (t >= nt) break;
             const long m = mb + 16 * t + r;
-            const v4s a = fragment(g.A, g.as_m, g.as_k, m, m < g.M, k, ke, avec);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            a[t] = fragment<AV>(g.A, g.as_m, g.as_k, m, m < g.M, k, ke);
         }
+    };
+    auto mfma = [&](const v4s (&a)[MT], const v4s &b) {
+#pragma unroll
+        for (int t = 0; t < MT; t++)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc[t], 0, 0, 0);
+    };
+#pragma unroll
+    for (int t = 0; t < MT; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    const long kq = 4 * q;
+    load(kb + kq, a0, b0);
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        load(k0 + 16 + kq, a1, b1);
+        mfma(a0, b0);
+        if (k0 + 16 >= ke) break;   // wave-uniform
+        load(k0 + 32 + kq, a0, b0);
+        mfma(a1, b1);
     }
     float *C = g.C + (long)blockIdx.z * g.M * g.ldc;
 #pragma unroll
@@ -124,13 +150,13 @@ template <int MT> __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 // operand B(k, n) = X[c][b][yo s + ky - p][xo s + kx - p] (k = (c KS + ky) KS
 // + kx, n = (b Ho + yo) Wo + xo) is gathered from the activations and
 // rounded to BF16 where k_gemm would read it from col: the same operands,
-// MFMAs and order, so the same results.
+// MFMAs and order, so the same results.  Pipelined like k_gemm.
 struct ConvGeom {
     const float *X;
     int B, H, W, stride, pad, Ho, Wo;
 };
 
-template <int MT, int KS>
+template <int MT, int KS, bool AV>
 __global__ __launch_bounds__(256) void k_gemm_conv(GemmArgs g, ConvGeom cg) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -148,32 +174,51 @@ __global__ __launch_bounds__(256) void k_gemm_conv(GemmArgs g, ConvGeom cg) {
     const long bstride = (long)cg.B * cg.H * cg.W;   // one channel
     const float *Xb = cg.X + (long)b * cg.H * cg.W;
     const int nt = min(MT, (g.M - mb + 15) >> 4);
-    const bool avec = g.as_k == 1 && (g.as_m & 3) == 0 && ((uintptr_t)g.A & 7) == 0 &&
-                      (ke & 3) == 0;
-    v4f acc[MT];
-#pragma unroll
-    for (int t = 0; t < MT; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        const int k = k0 + 4 * q;
-        v4s bf;
+    // the four activations of B's fragment for the k range starting at k
+    // (from kZeros for an element outside the image or past ke); they are
+    // rounded to BF16 one step later, so that the loads of the next step
+    // stay in flight while this step's MFMAs run
+    auto gather = [&](int k, float (&v)[4]) {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int kk = k + e;
             const int c = kk / (KS * KS), rr = kk - c * (KS * KS);
             const int ky = rr / KS, kx = rr - ky * KS;
             const int y = y0 + ky, x = x0 + kx;
-            float v = 0.0f;
-            if (nok && kk < ke && y >= 0 && y < cg.H && x >= 0 && x < cg.W)
-                v = Xb[c * bstride + y * cg.W + x];
-            bf[e] = (short)to_bf16(v);
+            const bool ok = nok && kk < ke && y >= 0 && y < cg.H && x >= 0 && x < cg.W;
+            v[e] = *(ok ? Xb + c * bstride + y * cg.W + x : (const float *)kZeros);
         }
+    };
+    v4f acc[MT];
+    // two sets in turn, as in k_gemm
+    v4s a0[MT], a1[MT];
+    float v0[4], v1[4];
+    auto load = [&](int k, v4s (&a)[MT], float (&v)[4]) {
+        gather(k, v);
 #pragma unroll
         for (int t = 0; t < MT; t++) {
-            if (t >= nt) break;
             const long m = mb + 16 * t + r;
-            const v4s a = fragment(g.A, g.as_m, g.as_k, m, m < g.M, k, ke, avec);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bf, acc[t], 0, 0, 0);
+            a[t] = fragment<AV>(g.A, g.as_m, g.as_k, m, m < g.M, k, ke);
         }
+    };
+    auto mfma = [&](const v4s (&a)[MT], const float (&v)[4]) {
+        v4s bf;
+#pragma unroll
+        for (int e = 0; e < 4; e++) bf[e] = (short)to_bf16(v[e]);
+#pragma unroll
+        for (int t = 0; t < MT; t++)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], bf, acc[t], 0, 0, 0);
+    };
+#pragma unroll
+    for (int t = 0; t < MT; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    const int kq = 4 * q;
+    load(kb + kq, a0, v0);
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        load(k0 + 16 + kq, a1, v1);
+        mfma(a0, v0);
+        if (k0 + 16 >= ke) break;   // wave-uniform
+        load(k0 + 32 + kq, a0, v0);
+        mfma(a1, v1);
     }
     float *C = g.C + (long)blockIdx.z * g.M * g.ldc;
 #pragma unroll
@@ -506,18 +551,32 @@ hipError_t gemm(lbm_nn n, int M, int N, int K, const bf16 *A, long as_m, long as
     const long blocks = ((M + 16L * mt - 1) / (16L * mt)) * tn;
     dim3 grid((unsigned)((blocks + 3) / 4), 1, (unsigned)splits);
     if (conv && ks != 1 && ks != 3) return hipErrorInvalidValue;   // kernels for 1x1 and 3x3
-    if (conv && ks == 3 && mt == 4)
-        hipLaunchKernelGGL((nn::k_gemm_conv<4, 3>), grid, dim3(256), 0, n->s, g, *conv);
-    else if (conv && ks == 3)
-        hipLaunchKernelGGL((nn::k_gemm_conv<1, 3>), grid, dim3(256), 0, n->s, g, *conv);
-    else if (conv && mt == 4)
-        hipLaunchKernelGGL((nn::k_gemm_conv<4, 1>), grid, dim3(256), 0, n->s, g, *conv);
-    else if (conv)
-        hipLaunchKernelGGL((nn::k_gemm_conv<1, 1>), grid, dim3(256), 0, n->s, g, *conv);
-    else if (mt == 4)
-        hipLaunchKernelGGL(nn::k_gemm<4>, grid, dim3(256), 0, n->s, g);
-    else
-        hipLaunchKernelGGL(nn::k_gemm<1>, grid, dim3(256), 0, n->s, g);
+    // 8-byte fragment loads where an operand's k axis is contiguous and
+    // aligned and every k range ends on a multiple of 4 (the ranges are
+    // multiples of 16 but the last, which ends at K)
+    const bool av = as_k == 1 && (as_m & 3) == 0 && ((uintptr_t)A & 7) == 0 && (K & 3) == 0;
+    const bool bv = bs_k == 1 && (bs_n & 3) == 0 && ((uintptr_t)B & 7) == 0 && (K & 3) == 0;
+    const dim3 block(256);
+    if (conv) {
+        auto c = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, 0, n->s, g, *conv); };
+        if (ks == 3 && mt == 4) av ? c(nn::k_gemm_conv<4, 3, true>) : c(nn::k_gemm_conv<4, 3, false>);
+        else if (ks == 3) av ? c(nn::k_gemm_conv<1, 3, true>) : c(nn::k_gemm_conv<1, 3, false>);
+        else if (mt == 4) av ? c(nn::k_gemm_conv<4, 1, true>) : c(nn::k_gemm_conv<4, 1, false>);
+        else av ? c(nn::k_gemm_conv<1, 1, true>) : c(nn::k_gemm_conv<1, 1, false>);
+    } else {
+        auto l = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, 0, n->s, g); };
+        if (mt == 4) {
+            if (av && bv) l(nn::k_gemm<4, true, true>);
+            else if (av) l(nn::k_gemm<4, true, false>);
+            else if (bv) l(nn::k_gemm<4, false, true>);
+            else l(nn::k_gemm<4, false, false>);
+        } else {
+            if (av && bv) l(nn::k_gemm<1, true, true>);
+            else if (av) l(nn::k_gemm<1, true, false>);
+            else if (bv) l(nn::k_gemm<1, false, true>);
+            else l(nn::k_gemm<1, false, false>);
+        }
+    }
     if (splits > 1)
         hipLaunchKernelGGL(nn::k_reduce_splits, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0,
                            n->s, (const float *)n->part, C, ldc, M, N, splits);
