@@ -52,14 +52,15 @@ CHECK_EVERY = 2000
 # The Nusselt stopping rule (settle_estimate): SETTLE_CHECKS hot-wall Nu
 # values, SETTLE_SPACING checks apart, ending at the latest check, bound the
 # drift still to come by SETTLE_TOL (relative), on SETTLE_HOLD checks in a
-# row.  Chosen on the traces of 100 held-out 257^2 cavities and the 128^3
-# cube, warm and cold, as a setting that never stopped more than 5.3e-4 from
-# the converged Nu (looser settings stopped up to 8e-3 away).
-SETTLE_TOL = 1e-4
-SETTLE_CHECKS = 6
-SETTLE_SPACING = 4
+# row.  Chosen on the traces of 108 held-out 257^2 cavities and the 128^3
+# cube, warm and cold: it never stopped more than 1.9e-4 from the converged
+# Nu.  The hold matters most: with a hold of two checks, a cold run whose Nu
+# flattened for about 20 checks between two modes was stopped 5.9e-3 away.
+SETTLE_TOL = 3e-5
+SETTLE_CHECKS = 4
+SETTLE_SPACING = 8
 SETTLE_SPREAD = 1.5
-SETTLE_HOLD = 2
+SETTLE_HOLD = 8
 WEIGHTS = os.path.join(_HERE, "data", "warm64.npz")
 # A cube larger than this starts from a coarse cube of this edge, itself
 # started from the extruded prediction and run to COARSE_EPS (warm_fields).

@@ -14,6 +14,7 @@ import os
 
 import pytest
 
+import lbmhip
 import surrogate as S
 
 pytestmark = pytest.mark.gpu
@@ -32,11 +33,17 @@ def _report(label, s):
           f"{s['band_wall_ratio']:.2f}x in wall time; stopping rule "
           f"{s['stop_itc_ratio']:.2f}x in steps, {s['stop_wall_ratio']:.2f}x in wall time; "
           f"largest converged-Nu deviation {s['nu_rel_dev_max']:.2e}", flush=True)
+    print(f"  {label}: Nusselt stopping rule {s['settle_itc_ratio']:.2f}x in steps, "
+          f"{s['settle_wall_ratio']:.2f}x in wall time (both runs stopped by it in "
+          f"{s['settle_by_nu']} of {s['settle_cavities']}); largest distance from the "
+          f"converged Nu at its stops {s['settle_nu_dev_max']:.2e}", flush=True)
 
 
 def test_c5_held_out_257_cavities(net257):
-    """Eight held-out 257^2 cavities (seed 7, the first eight of the bench's
-    hold-out set; log10 Ra ~ U[3, 6], Pr ~ U[0.7, 7], A ~ U[1, 2.5])."""
+    """Eight held-out 257^2 cavities (sample_cavities(8, 7): log10 Ra ~
+    U[3, 6], Pr ~ U[0.7, 7], A ~ U[1, 2.5]; the draws of each parameter come
+    in one block, so these share Ra with the first eight of the bench's 100
+    and have other Pr and A)."""
     cavs = S.sample_cavities(8, 7, n=257)
     print("\n[C5] 8 held-out 257^2 cavities, warm vs cold to eps 1e-6", flush=True)
     rows, s = S.compare(net257, cavs, itc_max=2000000)
@@ -50,6 +57,32 @@ def test_c5_held_out_257_cavities(net257):
     assert s["band_cavities"] == len(cavs)
     assert s["band_itc_ratio"] >= 3.0
     assert s["stop_itc_ratio"] >= 1.0
+    # the Nusselt stopping rule stops every run within 0.1% of the
+    # converged Nu
+    assert s["settle_cavities"] == len(cavs)
+    assert s["settle_nu_dev_max"] <= 1e-3
+
+
+def test_c5_nusselt_rule_run_live(net257):
+    """traced_run(stop="nu") on three held-out 257^2 cavities, cold and
+    warm: the run ends at the Nusselt rule's entry in the full run's trace
+    (the rule reads only earlier checks, and the trajectory is the same to
+    the bit), no later than the residual rule, within 0.1% of the converged
+    Nu."""
+    print("\n[C5] Nusselt stopping rule, run live", flush=True)
+    for cav in S.sample_cavities(3, 7, n=257):
+        for net in (None, net257):
+            full = S.traced_run(cav, net, itc_max=2000000)
+            live = S.traced_run(cav, net, itc_max=2000000, stop="nu")
+            ent = S.settle_entry(full["trace"])
+            want = int(ent[0]) if ent is not None else int(full["itc"])
+            dev = abs(live["nu"] / full["nu"] - 1.0)
+            print(f"  Ra {cav['Ra']:.3g} {'warm' if net else 'cold'}: residual rule at "
+                  f"{full['itc']}, Nusselt rule at {live['itc']}, Nu {dev:.1e} from the "
+                  f"converged value", flush=True)
+            assert full["status"] == lbmhip.LBM_OK and live["status"] == lbmhip.LBM_OK
+            assert live["itc"] == want and live["itc"] <= full["itc"]
+            assert dev <= 1e-3
 
 
 def test_c5_cube_128(net257):

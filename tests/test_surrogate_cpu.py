@@ -180,25 +180,51 @@ def test_settle_estimate_is_the_geometric_tail():
     assert S.settle_estimate(nan) == np.inf
 
 
-def test_settle_entry_is_the_first_check_where_the_rule_holds_twice():
+def test_settle_entry_is_the_first_check_where_the_rule_holds_long_enough():
     r, A = 0.93, 0.2
     k = np.arange(1, 241)
     nu = 4.0 - A * r ** k
     trace = np.stack([k * 2000, k * 0.25, nu], axis=1)
     est = np.array([S.settle_estimate(nu[:j]) for j in range(1, 241)])
     ok = est <= S.SETTLE_TOL
-    first = next(j for j in range(1, 240) if ok[j - 1] and ok[j])
+    H = S.SETTLE_HOLD
+    first = next(j for j in range(H - 1, 240) if ok[j - H + 1:j + 1].all())
     itc, wall, got = S.settle_entry(trace)
     assert (itc, wall, got) == (trace[first, 0], trace[first, 1], trace[first, 2])
     # the drift left at the stop is within the rule's bound
     assert abs(got / 4.0 - 1.0) <= S.SETTLE_TOL * 1.0001
-    # one check before, it did not yet hold twice
+    # one check before, it had not yet held SETTLE_HOLD checks in a row
     assert S.settle_entry(trace[:first]) is None
     # the span traced_run keeps is enough to decide at the latest check
     assert S.settle_entry(trace[first + 1 - S.SETTLE_SPAN:first + 1]) is not None
     assert S.settle_entry(trace[first - S.SETTLE_SPAN:first]) is None
     # a run that never settles
     assert S.settle_entry(np.stack([k, k, 1.0 + 0.01 * k], axis=1)) is None
+
+
+def test_settle_entry_does_not_stop_on_a_flat_stretch_between_two_modes(monkeypatch):
+    """A recorded cold run of a 257^2 cavity (tests/golden/nu_trace_257_flat.npz,
+    written by make_nu_trace_257_flat.py) whose Nu flattens for about 20
+    checks between two modes: the rule's setting does not stop it early
+    (it runs to the residual rule), while six values four checks apart with
+    a bound of 1e-4 held for two checks stop it on the flat stretch, 5.9e-3
+    away."""
+    import json
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "nu_trace_257_flat.npz"))
+    trace, nu = d["trace"], float(d["nu_converged"])
+    assert json.loads(str(d["cavity"]))["n"] == 257
+    # the rule's setting: no stop before the residual rule ends the run, or
+    # one within 0.1% of the converged Nu
+    got = S.settle_entry(trace)
+    assert got is None or abs(got[2] / nu - 1.0) <= 1e-3
+    # six values four checks apart, bound 1e-4, held two checks
+    monkeypatch.setattr(S, "SETTLE_CHECKS", 6)
+    monkeypatch.setattr(S, "SETTLE_SPACING", 4)
+    monkeypatch.setattr(S, "SETTLE_HOLD", 2)
+    early = S.settle_entry(trace, tol=1e-4)
+    assert early is not None and abs(early[2] / nu - 1.0) > 5e-3
+    assert got is None or early[0] < got[0]
 
 
 def _row(cs, ws, ci, wi, cb, wb, status=lbmhip.LBM_OK, fallback=False, dev=1e-5):
