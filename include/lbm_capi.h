@@ -311,10 +311,8 @@ int32_t lbm_batch_time_steps(lbm_batch b, int64_t nsteps, double *ms);
  *     nodes of every cavity, each held with a two-node halo in registers
  *     and LDS for two steps (the single-cavity variant 2); results equal
  *     variant 0's to the bit;
- *  5: one cooperative launch per lbm_batch_step call (up to 4096 steps per
- *     launch): as many workgroups as the device holds at once stride over
- *     the tiles of every cavity, with a grid-wide barrier between steps;
- *     MALL groups as in variant 0; results equal variant 0's to the bit. */
+ *  5: reserved (one cooperative launch per call with grid-wide barriers
+ *     between steps; switched off: rejected). */
 int32_t lbm_batch_set_launch(lbm_batch b, const lbm_launch *launch);
 int32_t lbm_batch_get_launch(lbm_batch b, lbm_launch *launch);
 int32_t lbm_batch_autotune(lbm_batch b, int32_t steps_per_trial, double *best_ms);

@@ -10,7 +10,12 @@ arithmetic compiled into another kernel) to FP32 rounding, within COOP_TOL.
 import numpy as np
 import pytest
 
-pytestmark = pytest.mark.gpu
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="variant 5 (the cooperative launch) is switched off: after three "
+                            "passing GPU runs, one run stopped making progress inside "
+                            "test_coop_batch_with_mall_groups and the cause is not established"),
+]
 
 # relative max-norm difference of the fields against single-cavity runs
 # (the FP32 field bar against the FP64 oracle is 1e-5)
